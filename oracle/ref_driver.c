@@ -8,11 +8,14 @@
  * gcge_oracle.c and serves as the "reference" CPU baseline.
  *
  * Usage: ref_solve <problem.bin> <out.bin> [-seed S] [-dirichlet 1]
- *                  [-gcge_<flag> <value> ...]
+ *                  [-cg_type T] [-cg_rate R] [-gcge_<flag> <value> ...]
  * The -gcge_* flags go to the reference's own parser
  * (GCGE_PARA_SetFromCommandLine, src/gcge_para.c:114-350) through
  * GCGE_CSR_Solver_Init (app/csr/gcge_app_csr.c:92-124).  When the problem
- * file carries X0, -gcge_given_init_evec 1 is implied.
+ * file carries X0, -gcge_given_init_evec 1 is implied.  cg_type has no
+ * command-line flag and -gcge_cg_rate is parsed with atoi
+ * (src/gcge_para.c:253), so -cg_type / -cg_rate set those fields directly
+ * after initialisation, as test/csr/test_csr_solver.c:48-50 does.
  *
  * Problem file (little endian): "GCGEPRB1", int32 n, nnzA, hasB, nnzB, nev,
  * hasX0, then rowptrA[n+1] (int32), colA[nnzA] (int32), valA[nnzA] (f64),
@@ -59,6 +62,8 @@ int main(int argc, char **argv)
 {
     char magic[8];
     int hdr[6], n, nnzA, hasB, nnzB, nev, hasX0, i, seed = 1, dirichlet = 0, nargs = 0;
+    int cg_type = -1;
+    double cg_rate = -1.0;
     char **args;
     FILE *f;
     CSR_MAT *A, *B = NULL;
@@ -90,6 +95,10 @@ int main(int argc, char **argv)
             seed = atoi(argv[++i]);
         } else if (strcmp(argv[i], "-dirichlet") == 0 && i + 1 < argc) {
             dirichlet = atoi(argv[++i]);
+        } else if (strcmp(argv[i], "-cg_type") == 0 && i + 1 < argc) {
+            cg_type = atoi(argv[++i]);
+        } else if (strcmp(argv[i], "-cg_rate") == 0 && i + 1 < argc) {
+            cg_rate = atof(argv[++i]);
         } else {
             args[nargs++] = argv[i];
         }
@@ -101,6 +110,8 @@ int main(int argc, char **argv)
     srand((unsigned)seed);
     solver = GCGE_CSR_Solver_Init(A, B, nev, nargs, args);
     solver->para->dirichlet_boundary = dirichlet;
+    if (cg_type >= 0) solver->para->cg_type = cg_type;
+    if (cg_rate >= 0.0) solver->para->cg_rate = cg_rate;
     if (hasX0) {
         CSR_VEC **ev = (CSR_VEC **)solver->evec;
         for (i = 0; i < nev; i++)
