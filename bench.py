@@ -1,0 +1,276 @@
+"""Benchmark: eigenpairs/s to 1e-8 relative residual, mixed precision vs FP64
+GCGE, on one MI355X.
+
+Workload (default): 3-D variable-coefficient diffusion on 160^3 cells
+(n = 4,096,000, 28.5M nonzeros), kappa = exp(g) with g a seeded Gaussian
+field (sigma 1, correlation length 8 cells), B = I, lowest nev = 100
+eigenpairs, X0 ~ U[-1,1] (seeded), every pair converged to
+||A x - lambda x|| / (|lambda| ||x||) <= 1e-8.
+
+A step is one complete mixed-precision solve (FP32 phase + FP64 refinement)
+from X0, with the matrix and X0 already in HBM.  After the timed steps one FP64
+solve from the same X0 is timed for the comparison.  The device kernels are
+timed with HIP events on the backend stream over the timed region; the CPU
+baseline is the reference built from its own sources (oracle/_ref/ref_solve)
+on one core, on a bounded sample (see ``cpu_baseline``).
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "gcge-1.0_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_problem(args):
+    from gcge_hip import problems as P
+    w = args.workload
+    if w == "vardiff3d":
+        A, B = P.vardiff3d(args.size, sigma=1.0, corr_len=8.0, seed=args.seed)
+    elif w == "lap2d":
+        A, B = P.lap2d(args.size)
+    elif w == "lap3d_mass":
+        A, B = P.lap3d_mass(args.size)
+    elif w == "vardiff2d":
+        A, B = P.vardiff2d(args.size, seed=args.seed)
+    elif w == "chung_lu":
+        A, B = P.chung_lu(args.size, seed=args.seed)
+    else:
+        raise SystemExit(f"unknown workload {w}")
+    return A, B
+
+
+def kernel_delta(after, before):
+    out = {}
+    for k, v in after.items():
+        b = before.get(k, dict(count=0, ms=0.0, bytes=0.0, flops=0.0))
+        out[k] = {f: v[f] - b[f] for f in v}
+    return out
+
+
+def roofline(stats, prec_peak_tflops):
+    """Dominant kernel class (largest device time) and its achieved rate."""
+    if not stats:
+        return None, None
+    name = max(stats, key=lambda k: stats[k]["ms"])
+    s = stats[name]
+    if s["ms"] <= 0:
+        return name, None
+    if name in ("gram", "lincomb"):
+        ach = s["flops"] / (s["ms"] * 1e-3) / 1e12
+        return name, dict(bound="mfma", achieved=round(ach, 3), peak=prec_peak_tflops, unit="TFLOP/s",
+                          frac=round(ach / prec_peak_tflops, 4), traffic=None)
+    ach = s["bytes"] / (s["ms"] * 1e-3) / 1e9
+    return name, dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                      frac=round(ach / HBM_PEAK_GBS, 4), traffic=None,
+                      launches=s["count"], avg_ms=round(s["ms"] / max(1, s["count"]), 4),
+                      bytes_per_launch=s["bytes"] / max(1, s["count"]))
+
+
+def cpu_baseline(args, n_full, nnz_full, fp64_iters, nev):
+    """Reference (oracle/_ref/ref_solve, one core) on a bounded sample: a few
+    outer iterations on the same generator at a reduced size with the same
+    nev; the per-iteration time is scaled linearly in the nonzero count to the
+    full problem and multiplied by the GPU FP64 iteration count."""
+    import oracle_api as O
+    from gcge_hip import problems as P
+    m = args.cpu_sample_size
+    its = args.cpu_sample_iters
+    if args.workload == "vardiff3d":
+        A, B = P.vardiff3d(m, seed=args.seed)
+    elif args.workload == "lap2d":
+        A, B = P.lap2d(m)
+    elif args.workload == "lap3d_mass":
+        A, B = P.lap3d_mass(m)
+    elif args.workload == "vardiff2d":
+        A, B = P.vardiff2d(m, seed=args.seed)
+    else:
+        A, B = P.chung_lu(m, seed=args.seed)
+    X0 = P.random_x0(A.shape[0], nev, args.seed + 1)
+    env = {"MKL_NUM_THREADS": "1", "OMP_NUM_THREADS": "1"}
+    kind = "reference"
+    t0 = time.perf_counter()
+    if O.ref_available():
+        r = O.ref_solve(A, B, nev, X0=X0, flags=["-gcge_ev_max_it", str(its), "-gcge_print_para", "0",
+                                                   "-gcge_print_result", "0"], env=env, timeout=600)
+        secs, iters = r["seconds"], r["info"]["num_iter"]
+    else:
+        kind = "port"
+        t1 = time.perf_counter()
+        o = O.solve(A, B, nev, X0=X0, ev_max_it=its)
+        secs, iters = time.perf_counter() - t1, o["info"]["num_iter"]
+    per_iter = secs / max(1, iters)
+    scale = (nnz_full + 2.0 * n_full) / (A.nnz + 2.0 * A.shape[0])
+    projected = per_iter * scale * max(1, fp64_iters)
+    log(f"cpu baseline: {kind} {iters} iterations at n={A.shape[0]} in {secs:.2f}s "
+        f"({time.perf_counter() - t0:.1f}s wall), projected {projected:.1f}s for the full solve")
+    return dict(value=nev / projected, unit="eigenpairs/s", cores=1, kind=kind,
+                sample=(f"{iters} outer iterations of the reference solve (1 core, MKL 1 thread) on "
+                        f"{args.workload} size {m} (n={A.shape[0]}, nnz={A.nnz}) with nev={nev}; "
+                        f"time per iteration scaled by (nnz+2n) to the full problem and multiplied by "
+                        f"the GPU FP64 iteration count {fp64_iters}"),
+                seconds_measured=round(secs, 3), projected_seconds=round(projected, 2))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="vardiff3d")
+    ap.add_argument("--size", type=int, default=160)
+    ap.add_argument("--nev", type=int, default=100)
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--max-it", type=int, default=2000)
+    ap.add_argument("--fp32-tol", type=float, default=1e-4)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-sample-size", type=int, default=24)
+    ap.add_argument("--cpu-sample-iters", type=int, default=3)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+                          "n_gpus": args.gpus}))
+        log("bench.py: only --gpus 1 is supported (one process drives one GPU)")
+        sys.exit(2)
+
+    from gcge_hip import Matrix, capi, solve
+    from gcge_hip import problems as P
+    L = capi.load()
+
+    t = time.perf_counter()
+    A, B = make_problem(args)
+    n, nnz, nev = A.shape[0], A.nnz, args.nev
+    X0 = P.random_x0(n, nev, args.seed + 1)
+    log(f"problem {args.workload} size {args.size}: n={n} nnz={nnz} nev={nev} ({time.perf_counter() - t:.1f}s)")
+    t = time.perf_counter()
+    mA = Matrix(A)
+    mB = Matrix(B) if B is not None else None
+    L.GCGE_HIP_Synchronize()
+    log(f"upload {time.perf_counter() - t:.2f}s")
+
+    common = dict(conv_type="NS", ev_tol=args.tol, ev_max_it=args.max_it)
+    mixed = dict(fp32_tol=args.fp32_tol, fp32_max_it=args.max_it, fp64_max_it=args.max_it)
+
+    def step(keep=False):
+        return solve(mA, mB, nev, X0=X0, precision="mixed", mixed=mixed, keep_evec=keep, **common)
+
+    for i in range(args.warmup):
+        r = step()
+        log(f"warmup {i}: {r['wall']:.2f}s  fp32 {r['info']['fp32_iters']} it / fp64 {r['info']['fp64_iters']} it, "
+            f"unconverged {r['num_unlock']}")
+    L.GCGE_HIP_SetProfiling(1)
+    before = capi.kernel_stats()
+    L.GCGE_HIP_Synchronize()
+    t0 = time.perf_counter()
+    results = []
+    for i in range(args.steps):
+        results.append(step(keep=(i == args.steps - 1)))
+        log(f"step {i}: {results[-1]['wall']:.2f}s")
+    L.GCGE_HIP_Synchronize()
+    t1 = time.perf_counter()
+    after = capi.kernel_stats()
+    L.GCGE_HIP_SetProfiling(0)
+    mixed_stats = kernel_delta(after, before)
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = nev / (ms_per_step * 1e-3)
+    last = results[-1]
+    converged = last["num_unlock"] == 0
+
+    # residual check of the mixed result on the device, FP64 (NS criterion)
+    from gcge_hip import MultiVec, Ops
+    ops = Ops()
+    xv = MultiVec(mA, nev, data=last["evec"])
+    rn, xn, bn = ops.residual_norms(mA, mB, xv, np.arange(nev), last["eval"])
+    ns = rn / (np.abs(last["eval"]) * bn)
+    xv.close()
+    ops.close()
+
+    fp64 = None
+    if not args.no_fp64:
+        L.GCGE_HIP_SetProfiling(1)
+        b64 = capi.kernel_stats()
+        r64 = solve(mA, mB, nev, X0=X0, precision="fp64", keep_evec=False, **common)
+        a64 = capi.kernel_stats()
+        L.GCGE_HIP_SetProfiling(0)
+        fp64 = dict(seconds=r64["wall"], iters=r64["num_iter"], unconverged=r64["num_unlock"],
+                    eval=r64["eval"], stats=kernel_delta(a64, b64))
+        log(f"fp64: {r64['wall']:.2f}s, {r64['num_iter']} it, unconverged {r64['num_unlock']}")
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(args, n, nnz + (B.nnz if B is not None else 0),
+                               fp64["iters"] if fp64 else last["info"]["fp64_iters"], nev)
+        except Exception as e:  # the baseline must not hide the GPU result
+            log(f"cpu baseline failed: {e}")
+
+    dom, roof = roofline(mixed_stats, FP32_PEAK_TFLOPS)
+    line = {
+        "metric": "Eigenpairs/sec to 1e-8 rel. residual, mixed vs FP64 GCGE, 1 MI355X",
+        "value": round(value, 4),
+        "unit": "eigenpairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload} size {args.size} (n={n}, nnz={nnz}), nev={nev}, B=I"
+                               if B is None else f"{args.workload} size {args.size} (n={n}, nnz={nnz}+{B.nnz}), nev={nev}",
+                   "n": n, "nnz": nnz, "nev": nev, "tol": args.tol,
+                   "criterion": "||Ax-lambda Bx||/(|lambda| ||Bx||)", "x0": "U[-1,1] seeded",
+                   "fp32_tol": args.fp32_tol},
+        "converged": bool(converged),
+        "max_ns_residual": float(np.max(ns)),
+        "mixed": dict(fp32_iters=last["info"]["fp32_iters"], fp64_iters=last["info"]["fp64_iters"],
+                      fp32_seconds=round(last["info"]["fp32_seconds"], 3),
+                      fp64_seconds=round(last["info"]["fp64_seconds"], 3)),
+        "roofline_kernel": dom,
+        "roofline": roof,
+        "kernels_mixed": {k: dict(count=v["count"], ms=round(v["ms"], 2),
+                                  gbs=round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else None)
+                          for k, v in mixed_stats.items() if v["count"]},
+        "cpu_baseline": cpu,
+    }
+    if fp64:
+        d64, roof64 = roofline(fp64["stats"], FP64_PEAK_TFLOPS)
+        line["fp64"] = dict(eigenpairs_per_s=round(nev / fp64["seconds"], 4), seconds=round(fp64["seconds"], 3),
+                            iters=fp64["iters"], converged=fp64["unconverged"] == 0,
+                            roofline_kernel=d64, roofline=roof64,
+                            kernels={k: dict(count=v["count"], ms=round(v["ms"], 2),
+                                             gbs=round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1)
+                                             if v["ms"] > 0 else None)
+                                     for k, v in fp64["stats"].items() if v["count"]})
+        line["mixed_vs_fp64"] = round(fp64["seconds"] / (ms_per_step * 1e-3), 3)
+        line["max_rel_eig_diff_mixed_vs_fp64"] = float(np.max(np.abs(last["eval"] - fp64["eval"]) /
+                                                           np.abs(fp64["eval"])))
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,1081 @@
+// gcge_core.cpp -- the GCG eigensolver driven through GCGE_OPS.
+//
+// Same algorithm, parameters and control flow as the reference's default
+// path (src/gcge_eigsol.c, gcge_xpw.c, gcge_cg.c, gcge_orthogonal.c,
+// gcge_rayleighritz.c, gcge_workspace.c, gcge_solver.c).  What changes is
+// how the large operations are issued: the reference loops over single
+// vectors (one MatDotVec / VecAxpby / VecInnerProd per column); here every
+// large-scale step is one block operation on device multivectors:
+//   - block CG: one SpMM+dot, one p-update and one fused x/r update per sweep
+//     for all active columns (GCGE_BCG, gcge_cg.c:137-302);
+//   - projection V2 -= V1 (V1^T B V2): one Gram and one fused linear
+//     combination (GCGE_SubOrthonormalization, gcge_orthogonal.c:1880-1929);
+//   - residuals: one fused kernel per batch of pairs
+//     (GCGE_CheckConvergence, gcge_eigsol.c:559-686).
+// The small dense work (subspace matrices, in-subspace orthogonalization,
+// dsyevx) stays on the host in FP64 and calls the same LAPACK routines as the
+// reference.  Where the reference exchanges vector pointers between V and
+// evec/V_tmp (MultiVecSwap), this code copies: the columns that the swap
+// would hand back to evec/V_tmp are never read before being overwritten.
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "gcge.h"
+
+namespace {
+
+double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct Sec {
+    double *acc_total, *acc_iter, t0;
+    Sec(double *tot, double *it) : acc_total(tot), acc_iter(it), t0(now_s()) {}
+    ~Sec()
+    {
+        double d = now_s() - t0;
+        *acc_total += d;
+        *acc_iter = d;
+    }
+};
+#define GCGE_TIMER(name, para) Sec _sec_##name(&(para)->stat_para->part_time_total->name, \
+                                               &(para)->stat_para->part_time_one_iter->name)
+
+inline void set2(GCGE_INT *a, int x, int y)
+{
+    a[0] = x;
+    a[1] = y;
+}
+
+// ------------------------------------------------------------------------
+// Orthonormalization (src/gcge_orthogonal.c)
+// ------------------------------------------------------------------------
+
+// GCGE_Orthonormalization, gcge_orthogonal.c:112-305.  Block version: the
+// B-inner products of column `current` with columns 0..current come from
+// one Gram call, the projection is one fused combination.
+void orth_gs(void **V, int start, int *end, void *B, GCGE_OPS *ops, GCGE_PARA *para, void **V_tmp,
+             double *d_tmp)
+{
+    GCGE_ORTH_PARA *op = para->orth_para;
+    GCGE_INT ms[2], me[2];
+    double vin = 0.0, vout = 0.0, norm_value = 0.0, ipv = 0.0, ratio = 1.0;
+    const double sum_res = para->sum_res, crit = op->criterion_tol;
+    for (int current = start; current < *end; ++current) {
+        int reorth_count = 0;
+        if (sum_res > 0.0 && sum_res < crit) {
+            // column version, :139-208 (its reorthogonalization counter is
+            // never incremented in the reference; capped here)
+            void *vc, *t2 = nullptr;
+            ops->GetVecFromMultiVec(V, current, &vc);
+            if (B) ops->GetVecFromMultiVec(V_tmp, 1, &t2);
+            if (!B) {
+                ops->VecInnerProd(vc, vc, &norm_value);
+            } else {
+                ops->MatDotVec(B, vc, t2);
+                ops->VecInnerProd(vc, t2, &norm_value);
+            }
+            vout = std::sqrt(norm_value);
+            int guard = 0;
+            do {
+                for (int j = 0; j < current; j++) {
+                    void *vj;
+                    ops->GetVecFromMultiVec(V, j, &vj);
+                    if (!B) {
+                        ops->VecInnerProd(vc, vj, &ipv);
+                    } else {
+                        ops->MatDotVec(B, vc, t2);
+                        ops->VecInnerProd(vj, t2, &ipv);
+                    }
+                    ops->VecAxpby(-ipv, vj, 1.0, vc);
+                }
+                vin = vout;
+                if (!B) {
+                    ops->VecInnerProd(vc, vc, &norm_value);
+                } else {
+                    ops->MatDotVec(B, vc, t2);
+                    ops->VecInnerProd(vc, t2, &norm_value);
+                }
+                vout = std::sqrt(norm_value);
+                ratio = vout / vin;
+                guard++;
+            } while (ratio < op->reorth_tol && reorth_count < op->max_reorth_time && vout > op->orth_zero_tol &&
+                     guard < 1000);
+        } else {
+            // block version, :209-273
+            do {
+                void **T = V_tmp;
+                int tc = 0;
+                if (!B) {
+                    T = V;
+                    tc = current;
+                } else {
+                    set2(ms, current, 0);
+                    set2(me, current + 1, 1);
+                    ops->MatDotMultiVec(B, V, V_tmp, ms, me, ops);
+                }
+                set2(ms, 0, tc);
+                set2(me, current + 1, tc + 1);
+                ops->MultiVecInnerProd(V, T, d_tmp, (char *)"ns", ms, me, current + 1, ops);
+                if (current != 0) {
+                    set2(ms, 0, current);
+                    set2(me, current, current + 1);
+                    ops->MultiVecLinearCombAxpby(V, V, ms, me, d_tmp, current, -1.0, 1.0, ops);
+                }
+                vin = std::sqrt(d_tmp[current]);
+                d_tmp[current] = d_tmp[current] - ops->ArrayDotArray(d_tmp, d_tmp, current);
+                vout = std::sqrt(d_tmp[current]);
+                ratio = vout / vin;
+                reorth_count++;
+            } while (ratio < op->reorth_tol && reorth_count < op->max_reorth_time && vout > op->orth_zero_tol);
+        }
+        if (vout > op->orth_zero_tol) {
+            ops->MultiVecAxpbyColumn(0.0, V, current, 1 / vout, V, current, ops);
+        } else {
+            set2(ms, current, *end - 1);
+            set2(me, current + 1, *end);
+            ops->MultiVecSwap(V, V, ms, me, ops);
+            (*end)--;
+            current--;
+            if (op->print_orth_zero == 1)
+                printf("In Orthonormalization, there is a zero vector!, current = %d, start = %d, end = %d\n",
+                       current, start, *end);
+        }
+    }
+}
+
+// GCGE_SubOrthonormalization, gcge_orthogonal.c:1880-1929:
+// V2 = V2 - V1 (V1^T B V2), as one Gram and one fused combination.
+void sub_orth(void **V, const GCGE_INT *start, const GCGE_INT *end, void *B, void **V_tmp, double *d, GCGE_OPS *ops)
+{
+    const int len1 = end[0] - start[0], len2 = end[1] - start[1];
+    GCGE_INT ms[2], me[2];
+    void **T = V;
+    int tc = start[1];
+    if (B) {
+        set2(ms, start[1], 0);
+        set2(me, end[1], len2);
+        ops->MatDotMultiVec(B, V, V_tmp, ms, me, ops);
+        T = V_tmp;
+        tc = 0;
+    }
+    set2(ms, start[0], tc);
+    set2(me, end[0], tc + len2);
+    ops->MultiVecInnerProd(V, T, d, (char *)"ns", ms, me, len1, ops);
+    set2(ms, start[0], start[1]);
+    set2(me, end[0], end[1]);
+    ops->MultiVecLinearCombAxpby(V, V, ms, me, d, len1, -1.0, 1.0, ops);
+}
+
+// GCGE_SubOrthonormalizationSelfBGS, gcge_orthogonal.c:2035-2115.
+void self_bgs(void **V, int start, int *end, void *B, GCGE_PARA *para, GCGE_OPS *ops, GCGE_WORKSPACE *ws)
+{
+    GCGE_INT ms[2], me[2];
+    double *d = ws->subspace_dtmp;
+    const double tol = para->orth_para->orth_zero_tol;
+    std::vector<double> coef;
+    for (int current = start; current < *end; current++) {
+        void **T = V;
+        int tc = current;
+        if (B) {
+            set2(ms, current, 0);
+            set2(me, current + 1, 1);
+            ops->MatDotMultiVec(B, V, ws->V_tmp, ms, me, ops);
+            T = ws->V_tmp;
+            tc = 0;
+        }
+        set2(ms, current, tc);
+        set2(me, *end, tc + 1);
+        const int length = *end - current;
+        ops->MultiVecInnerProd(V, T, d, (char *)"ns", ms, me, length, ops);
+        const double norm_value = std::sqrt(d[0]);
+        if (norm_value > tol) {
+            ops->MultiVecAxpbyColumn(0.0, V, current, 1.0 / norm_value, V, current, ops);
+            if (current + 1 < *end) {
+                // V[j] += (-d[j-current]/norm) V[current], j > current
+                coef.assign(*end - current - 1, 0.0);
+                for (int j = current + 1; j < *end; j++) coef[j - current - 1] = -d[j - current] / norm_value;
+                set2(ms, current, current + 1);
+                set2(me, current + 1, *end);
+                ops->MultiVecLinearCombAxpby(V, V, ms, me, coef.data(), 1, 1.0, 1.0, ops);
+            }
+        } else {
+            set2(ms, current, (*end) - 1);
+            set2(me, current + 1, *end);
+            ops->MultiVecSwap(V, V, ms, me, ops);
+            (*end)--;
+            current--;
+            if (para->orth_para->print_orth_zero == 1)
+                printf("In GCGE_SubOrthonormalizationSelfBGS, there is a zero vector!, current = %d, start = %d, "
+                       "end = %d\n",
+                       current, start, *end);
+        }
+    }
+}
+
+// GCGE_MultiOrthonormalization, gcge_orthogonal.c:1945-2033.
+void multi_orth(void **V, int start, int *end, void *B, GCGE_OPS *ops, GCGE_PARA *para, GCGE_WORKSPACE *ws)
+{
+    const int length = *end - start;
+    const int length_1 = length / 2, length_2 = length - length_1;
+    int mid = start + length_1;
+    GCGE_INT ms[2], me[2];
+    const double tol = para->orth_para->scbgs_reorth_tol;
+    if (length <= para->orth_para->max_direct_orth_length) {
+        self_bgs(V, start, end, B, para, ops, ws);
+        return;
+    }
+    multi_orth(V, start, &mid, B, ops, para, ws);
+    if (mid + length_2 < *end) {
+        const int new_end_2 = mid + length_2, n_zero = *end - new_end_2;
+        set2(ms, mid, new_end_2);
+        set2(me, mid + n_zero, *end);
+        ops->MultiVecSwap(V, V, ms, me, ops);
+        *end = new_end_2;
+    }
+    if (mid - start > 0) {
+        double value_inner = 1.0;
+        int iter = 0;
+        while (value_inner > tol) {
+            iter++;
+            set2(ms, start, mid);
+            set2(me, mid, *end);
+            sub_orth(V, ms, me, B, ws->V_tmp, ws->subspace_dtmp, ops);
+            // the reference sums length_1*length_2 entries with the
+            // pre-deflation length_1 (gcge_orthogonal.c:2014)
+            value_inner = 0.0;
+            for (int i = 0; i < length_1 * length_2; i++) value_inner += ws->subspace_dtmp[i] * ws->subspace_dtmp[i];
+            value_inner = std::sqrt(value_inner);
+            if (iter >= para->orth_para->max_reorth_time) break;
+        }
+        if (value_inner > tol && para->print_eval == 1)
+            printf("in GCGE_SubOrthonormalization, V1: %d-%d, V2: %d-%d, reorth_count: %d, value_inner: %e\n", start,
+                   mid - 1, mid, *end, para->orth_para->max_reorth_time, value_inner);
+    }
+    multi_orth(V, mid, end, B, ops, para, ws);
+}
+
+// GCGE_StableMultiOrthonormalization, gcge_orthogonal.c:1833-1877.
+void stable_multi_orth(void **V, int start, int *end, void *B, GCGE_OPS *ops, GCGE_PARA *para, GCGE_WORKSPACE *ws)
+{
+    GCGE_INT ms[2], me[2];
+    double value_inner = 1.0;
+    int iter = 0;
+    set2(ms, 0, start);
+    set2(me, start, *end);
+    while (value_inner > para->orth_para->scbgs_reorth_tol) {
+        iter++;
+        sub_orth(V, ms, me, B, ws->V_tmp, ws->subspace_dtmp, ops);
+        value_inner = 0.0;
+        for (int i = 0; i < start * (*end - start); i++) value_inner += ws->subspace_dtmp[i] * ws->subspace_dtmp[i];
+        value_inner = std::sqrt(value_inner);
+        if (iter >= para->orth_para->max_reorth_time) break;
+    }
+    multi_orth(V, start, end, B, ops, para, ws);
+}
+
+// GCGE_MultiOrthonormalizationInSubspace, gcge_orthogonal.c:2130-2204
+// (host FP64; the second product updates ldV rows as the reference's does).
+void multi_orth_insub(double *Vd, int ldV, int nrows, int start, int *end, GCGE_ORTH_PARA *op, GCGE_WORKSPACE *ws,
+                      GCGE_OPS *ops)
+{
+    const int length = *end - start;
+    int length_1 = length / 2;
+    const int length_2 = length - length_1;
+    int mid = start + length_1;
+    if (length == 1) {
+        const double v = ops->ArrayNorm(Vd + (size_t)start * ldV, nrows);
+        if (v > op->orth_zero_tol) ops->ArrayScale(1.0 / v, Vd + (size_t)start * ldV, nrows);
+        else *end = start;
+        return;
+    }
+    multi_orth_insub(Vd, ldV, nrows, start, &mid, op, ws, ops);
+    if (mid + length_2 < *end) {
+        const int new_end_2 = mid + length_2, n_zero = *end - new_end_2;
+        length_1 -= n_zero;
+        std::memcpy(Vd + (size_t)mid * ldV, Vd + (size_t)new_end_2 * ldV, (size_t)n_zero * ldV * sizeof(double));
+        *end = new_end_2;
+    }
+    if (length_1 > 0) {
+        for (int rc = 0; rc < op->max_reorth_time; rc++) {
+            double alpha = 1.0, beta = 0.0;
+            int l1 = length_1, l2 = length_2, nr = nrows, ld = ldV;
+            ops->DenseMatDotDenseMat((char *)"T", (char *)"N", &l1, &l2, &nr, &alpha, Vd + (size_t)start * ldV, &ld,
+                                     Vd + (size_t)mid * ldV, &ld, &beta, ws->subspace_dtmp, &l1);
+            alpha = -1.0;
+            beta = 1.0;
+            ops->DenseMatDotDenseMat((char *)"N", (char *)"N", &ld, &l2, &l1, &alpha, Vd + (size_t)start * ldV, &ld,
+                                     ws->subspace_dtmp, &l1, &beta, Vd + (size_t)mid * ldV, &ld);
+        }
+    }
+    multi_orth_insub(Vd, ldV, nrows, mid, end, op, ws, ops);
+}
+
+// GCGE_StableMultiOrthonormalizationInSubspace, gcge_orthogonal.c:2215-2247.
+void stable_multi_orth_insub(double *Vd, int ldV, int nrows, int start, int *end, GCGE_ORTH_PARA *op,
+                             GCGE_WORKSPACE *ws, GCGE_OPS *ops)
+{
+    if (start != 0) {
+        for (int rc = 0; rc < op->max_reorth_time; rc++) {
+            int len = *end - start, st = start, nr = nrows, ld = ldV;
+            double alpha = 1.0, beta = 0.0;
+            ops->DenseMatDotDenseMat((char *)"T", (char *)"N", &st, &len, &nr, &alpha, Vd, &ld,
+                                     Vd + (size_t)start * ldV, &ld, &beta, ws->subspace_dtmp, &st);
+            alpha = -1.0;
+            beta = 1.0;
+            ops->DenseMatDotDenseMat((char *)"N", (char *)"N", &ld, &len, &st, &alpha, Vd, &ld, ws->subspace_dtmp, &st,
+                                     &beta, Vd + (size_t)start * ldV, &ld);
+        }
+    }
+    multi_orth_insub(Vd, ldV, nrows, start, end, op, ws, ops);
+}
+
+// ------------------------------------------------------------------------
+// CG solvers (src/gcge_cg.c)
+// ------------------------------------------------------------------------
+
+// GCGE_CG, gcge_cg.c:26-113 (only with if_use_bcg != 1), per-vector ops.
+void cg_single(void *A, void *b, void *x, GCGE_OPS *ops, GCGE_PARA *para, void **V_tmp1, void **V_tmp2)
+{
+    void *r, *p, *w;
+    int niter, max_it = para->cg_max_it, type = para->cg_type;
+    double rate = para->cg_rate, alpha, beta, rho1 = 0.0, rho2, error, last_error, pTw, wTw;
+    ops->GetVecFromMultiVec(V_tmp1, 1, &r);
+    ops->GetVecFromMultiVec(V_tmp2, 0, &p);
+    ops->GetVecFromMultiVec(V_tmp2, 1, &w);
+    ops->MatDotVec(A, x, r);
+    ops->VecAxpby(1.0, b, -1.0, r);
+    ops->VecInnerProd(r, r, &rho2);
+    error = std::sqrt(rho2);
+    niter = 1;
+    last_error = error;
+    while ((last_error / error >= rate) && (niter < max_it)) {
+        if (niter == 1) {
+            ops->VecAxpby(1.0, r, 0.0, p);
+        } else {
+            beta = rho2 / rho1;
+            ops->VecAxpby(1.0, r, beta, p);
+        }
+        ops->MatDotVec(A, p, w);
+        if (type == 1) {
+            ops->VecInnerProd(p, w, &pTw);
+            alpha = rho2 / pTw;
+            ops->VecAxpby(alpha, p, 1.0, x);
+            ops->VecAxpby(-alpha, w, 1.0, r);
+            rho1 = rho2;
+            ops->VecInnerProd(r, r, &rho2);
+        } else {
+            ops->VecInnerProd(p, w, &pTw);
+            ops->VecInnerProd(w, w, &wTw);
+            alpha = rho2 / pTw;
+            ops->VecAxpby(alpha, p, 1.0, x);
+            ops->VecAxpby(-alpha, w, 1.0, r);
+            rho1 = rho2;
+            rho2 = rho1 - 2.0 * alpha * pTw + alpha * alpha * wTw;
+        }
+        last_error = std::sqrt(rho2);
+        niter++;
+    }
+}
+
+// GCGE_BCG, gcge_cg.c:137-302, one block operation per step for all active
+// columns.  The reference rebuilds its active list with unlock[idx] = idx
+// (:289-293) starting from the identity, so after each sweep the active
+// columns are 0..num_unlock-1 -- the first num_unlock columns, not the
+// unconverged ones.  That rule is kept.
+void bcg(void *A, void **RHS, void **V, int x_start, int x_length, GCGE_OPS *ops, GCGE_PARA *para, void **CG_P,
+         void **CG_W, double *dtmp, int *itmp)
+{
+    if (x_length <= 0) return;
+    double *rho1 = dtmp, *rho2 = rho1 + x_length, *error = rho2 + x_length;
+    double *last_error = error + x_length, *ptw = last_error + x_length;
+    int *unlock = itmp;
+    const int max_it = para->cg_max_it;
+    const double rate = para->cg_rate;
+    GCGE_INT ms[2], me[2];
+    std::vector<double> a(x_length), b(x_length), alpha(x_length), rr(x_length);
+
+    set2(ms, x_start, 0);
+    set2(me, x_start + x_length, x_length);
+    ops->MatDotMultiVec(A, V, CG_W, ms, me, ops);
+    set2(ms, 0, 0);
+    set2(me, x_length, x_length);
+    ops->MultiVecAxpby(-1.0, CG_W, 1.0, RHS, ms, me, ops);
+    ops->MultiVecInnerProdDiag(RHS, RHS, rho2, ms, me, ops);
+    for (int idx = 0; idx < x_length; idx++) unlock[idx] = idx;
+    int num_unlock = x_length;
+    for (int idx = 0; idx < x_length; idx++) error[idx] = std::sqrt(rho2[idx]);
+    int niter = 1;
+    while (num_unlock > 0 && niter < max_it) {
+        const int na = num_unlock;  // active columns are unlock[0..na) = 0..na-1
+        set2(ms, 0, 0);
+        set2(me, na, na);
+        if (niter == 1) {
+            ops->MultiVecAxpby(1.0, RHS, 0.0, CG_P, ms, me, ops);
+        } else {
+            for (int id = 0; id < na; id++) {
+                a[id] = 1.0;
+                b[id] = rho2[id] / rho1[id];
+            }
+            ops->MultiVecAxpbyDiag(a.data(), RHS, b.data(), CG_P, ms, me, ops);
+        }
+        ops->MatDotMultiVecInnerProdDiag(A, CG_P, CG_W, ms, me, ptw, ops);
+        for (int id = 0; id < na; id++) {
+            alpha[id] = rho2[id] / ptw[id];
+            rho1[id] = rho2[id];
+        }
+        ops->BlockCGUpdate(V, x_start, CG_P, CG_W, RHS, na, alpha.data(), rr.data(), ops);
+        for (int id = 0; id < na; id++) rho2[id] = rr[id];
+        for (int idx = 0; idx < x_length; idx++) last_error[idx] = std::sqrt(std::fabs(rho2[idx]));
+        num_unlock = 0;
+        for (int idx = 0; idx < x_length; idx++) {
+            if (last_error[idx] / error[idx] >= rate) {
+                unlock[idx] = idx;
+                num_unlock++;
+            }
+        }
+        niter++;
+    }
+    para->bcg_calls++;
+    para->bcg_iters_total += niter - 1;
+}
+
+// ------------------------------------------------------------------------
+// X, W, P (src/gcge_xpw.c)
+// ------------------------------------------------------------------------
+
+// GCGE_ComputeX, gcge_xpw.c:47-76.
+void compute_x(void **V, double *coef, void **X, int size_X, int size_V, GCGE_OPS *ops, GCGE_WORKSPACE *ws)
+{
+    GCGE_INT s[2], e[2];
+    const int sx = ws->num_soft_locked_in_last_iter;
+    set2(s, sx, sx);
+    set2(e, size_V, size_X);
+    ops->MultiVecLinearComb(V, X, s, e, coef + (size_t)sx * size_V + sx, size_V, nullptr, 0, ops);
+}
+
+// GCGE_ComputeW, gcge_xpw.c:118-217: W_j starts from x_j (|lambda| <= 1) or
+// x_j / lambda; the right-hand side is lambda B x_j or B x_j.
+void compute_w(void *A, void *B, void **V, double *eval, GCGE_OPS *ops, GCGE_PARA *para, GCGE_WORKSPACE *ws)
+{
+    const int w_start = ws->dim_xp, w_length = ws->unconv_bs;
+    if (w_length <= 0) return;
+    GCGE_INT ms[2], me[2];
+    std::vector<double> ra(w_length, 0.0), rb(w_length), wa(w_length, 0.0), wb(w_length);
+    // W_j <- x_{unlock[j]}
+    ops->MultiVecGather(V, ws->unlock, w_length, V, w_start, ops);
+    set2(ms, w_start, 0);
+    set2(me, w_start + w_length, w_length);
+    if (!B) ops->MultiVecAxpby(1.0, V, 0.0, ws->V_tmp, ms, me, ops);
+    else ops->MatDotMultiVec(B, V, ws->V_tmp, ms, me, ops);
+    bool scale_rhs = false, scale_w = false;
+    for (int j = 0; j < w_length; j++) {
+        const double lam = eval[ws->unlock[j]];
+        if (std::fabs(lam) <= 1.0) {
+            rb[j] = lam;  // rhs = lambda * rhs
+            wb[j] = 1.0;
+            scale_rhs = true;
+        } else {
+            rb[j] = 1.0;
+            wb[j] = 1.0 / lam;  // w = x / lambda
+            scale_w = true;
+        }
+    }
+    set2(ms, 0, 0);
+    set2(me, w_length, w_length);
+    if (scale_rhs) ops->MultiVecAxpbyDiag(ra.data(), ws->V_tmp, rb.data(), ws->V_tmp, ms, me, ops);
+    set2(ms, w_start, w_start);
+    set2(me, w_start + w_length, w_start + w_length);
+    if (scale_w) ops->MultiVecAxpbyDiag(wa.data(), V, wb.data(), V, ms, me, ops);
+
+    if (ops->LinearSolver) {
+        for (int idx = 0; idx < w_length; idx++) {
+            void *rhs, *wc;
+            ops->GetVecFromMultiVec(ws->V_tmp, idx, &rhs);
+            ops->GetVecFromMultiVec(V, w_start + idx, &wc);
+            ops->LinearSolver(A, rhs, wc, ops);
+            ops->RestoreVecForMultiVec(V, w_start + idx, &wc);
+            ops->RestoreVecForMultiVec(ws->V_tmp, idx, &rhs);
+        }
+    } else if (para->if_use_bcg == 1) {
+        bcg(A, ws->V_tmp, V, w_start, w_length, ops, para, ws->CG_p, ws->evec, ws->subspace_dtmp, ws->subspace_itmp);
+    } else {
+        for (int idx = 0; idx < w_length; idx++) {
+            void *rhs, *wc;
+            ops->GetVecFromMultiVec(ws->V_tmp, idx, &rhs);
+            ops->GetVecFromMultiVec(V, w_start + idx, &wc);
+            cg_single(A, rhs, wc, ops, para, ws->V_tmp, ws->evec);
+        }
+    }
+}
+
+// GCGE_ComputeP, gcge_xpw.c:286-425 ("multi" subspace orthogonalization).
+void compute_p(double *coef, void **V, GCGE_OPS *ops, GCGE_PARA *para, GCGE_WORKSPACE *ws)
+{
+    GCGE_TIMER(p_orth_time, para);
+    const int ldc = ws->dim_xpw;
+    int p_start = ws->dim_x, p_ncols = ws->unconv_bs;
+    const int xx_nrows = ws->last_dim_x, nev = para->nev;
+    GCGE_INT ms[2], me[2];
+    double *px = coef + (size_t)p_start * ldc, *pp = px + xx_nrows, *xp = coef + xx_nrows;
+    std::memset(px, 0, (size_t)p_ncols * ldc * sizeof(double));
+    for (int cp = 0; cp < p_ncols; cp++) {
+        const int cx = ws->unlock[cp];
+        std::memcpy(pp + (size_t)cp * ldc, xp + (size_t)cx * ldc, (size_t)(ldc - xx_nrows) * sizeof(double));
+    }
+    const int orth_start = ws->num_soft_locked_in_last_iter;
+    const int orth_length = ldc - orth_start;
+    p_start -= orth_start;
+    int p_end = p_start + p_ncols;
+    stable_multi_orth_insub(coef + (size_t)orth_start * ldc + orth_start, ldc, orth_length, p_start, &p_end,
+                            para->orth_para, ws, ops);
+    p_end += orth_start;
+    const int p_end_in_V_tmp = p_end - nev;
+    {
+        GCGE_TIMER(p_axpy_time, para);
+        if (p_end_in_V_tmp > 0) {
+            set2(ms, orth_start, 0);
+            set2(me, ldc, p_end_in_V_tmp);
+            ops->MultiVecLinearComb(V, ws->V_tmp, ms, me, coef + (size_t)nev * ldc + orth_start, ldc, nullptr, -1,
+                                    ops);
+            set2(ms, 0, nev);
+            set2(me, p_end_in_V_tmp, p_end);
+            ops->MultiVecCopy(ws->V_tmp, V, ms, me, ops);
+        }
+    }
+    ws->dim_xp = p_end;
+}
+
+// ------------------------------------------------------------------------
+// Rayleigh-Ritz (src/gcge_rayleighritz.c)
+// ------------------------------------------------------------------------
+
+// GCGE_ComputeSubspaceMatrixVTAW, gcge_rayleighritz.c:35-81.
+void rr_vtaw(void **V, void *A, int start_W, int end_V, double *mat, GCGE_OPS *ops, void **work)
+{
+    const int w_length = end_V - start_W, xp_length = end_V - w_length;
+    GCGE_INT ms[2], me[2];
+    set2(ms, start_W, 0);
+    set2(me, end_V, w_length);
+    ops->MatDotMultiVec(A, V, work, ms, me, ops);
+    set2(ms, 0, 0);
+    set2(me, xp_length, w_length);
+    ops->MultiVecInnerProd(V, work, mat, (char *)"nonsym", ms, me, end_V, ops);
+    set2(ms, start_W, 0);
+    set2(me, end_V, w_length);
+    ops->MultiVecInnerProd(V, work, mat + xp_length, (char *)"sym", ms, me, end_V, ops);
+}
+
+// GCGE_ComputeSubspaceMatrix, gcge_rayleighritz.c:113-200 (live branch).
+void rr_matrix(void *A, void **V, GCGE_OPS *ops, GCGE_PARA *para, GCGE_WORKSPACE *ws)
+{
+    GCGE_TIMER(rr_mat_time, para);
+    int dim_xp = ws->dim_xp, ldm = ws->dim_xpw;
+    if (dim_xp != 0) {
+        int lde = ws->last_dim_xpw, dim_x = ws->dim_x, dim_p = dim_xp - dim_x;
+        double alpha = 1.0, beta = 0.0;
+        double *pp = ws->subspace_evec + (size_t)dim_x * lde;
+        double *AApp = ws->subspace_dtmp;
+        double *PTAP = ws->subspace_matrix + (size_t)dim_x * ldm + dim_x;
+        ops->DenseSymMatDotDenseMat((char *)"L", (char *)"U", &lde, &dim_p, &alpha, ws->subspace_matrix, &lde, pp,
+                                    &lde, &beta, AApp, &lde);
+        std::memset(ws->subspace_matrix, 0, (size_t)ldm * ldm * sizeof(double));
+        ops->DenseMatDotDenseMat((char *)"T", (char *)"N", &dim_p, &dim_p, &lde, &alpha, pp, &lde,
+                                 ws->subspace_dtmp, &lde, &beta, PTAP, &ldm);
+        for (int i = 0; i < dim_x; i++) ws->subspace_matrix[(size_t)i * ldm + i] = ws->eval[i];
+    }
+    rr_vtaw(V, A, dim_xp, ldm, ws->subspace_matrix + (size_t)dim_xp * ldm, ops, ws->evec);
+}
+
+// GCGE_ComputeSubspaceEigenpairs, gcge_rayleighritz.c:364-590 (one process).
+void rr_eigenpairs(double *subspace_matrix, double *eval, double *subspace_evec, GCGE_OPS *ops, GCGE_PARA *para,
+                   GCGE_WORKSPACE *ws)
+{
+    GCGE_TIMER(rr_eigen_time, para);
+    int ldm = ws->dim_xpw, info = 0;
+    double *temp = ws->subspace_dtmp, *dwork = temp + (size_t)ldm * ldm;
+    double vl = 0.0, vu = 0.0, abstol = 0.0;
+    int rr_eigen_start = ws->unlock[0];
+    if (para->opt_rr_eig_partly == 0) rr_eigen_start = 0;
+    int il = rr_eigen_start;
+    for (int i = rr_eigen_start - 1; i > -1; i--)
+        if (std::fabs(eval[i] - eval[rr_eigen_start]) / eval[rr_eigen_start] < para->multi_tol_for_lock) il -= 1;
+    rr_eigen_start = il;
+    il += 1;
+    ws->num_soft_locked_in_last_iter = rr_eigen_start;
+    int iu = (ws->max_dim_x < ldm) ? ws->max_dim_x : ldm;
+    int m = iu - il + 1, lwork = 8 * ldm, liwork = 5 * ldm;
+    int *isuppz = ws->subspace_itmp + 10 * ldm, *ifail = ws->subspace_itmp + 5 * ldm;
+    for (int i = 0; i < ws->dim_xpw - ws->dim_xp; i++)
+        std::memset(subspace_matrix + (size_t)(ws->dim_xp + i) * ldm, 0, (size_t)rr_eigen_start * sizeof(double));
+    std::memcpy(temp, subspace_matrix, (size_t)ldm * ldm * sizeof(double));
+    int nrows = ldm - rr_eigen_start, ncols = ldm;
+    il -= rr_eigen_start;
+    iu -= rr_eigen_start;
+    ops->DenseMatEigenSolver((char *)"V", (char *)"I", (char *)"U", &nrows,
+                             temp + (size_t)rr_eigen_start * ncols + rr_eigen_start, &ncols, &vl, &vu, &il, &iu,
+                             &abstol, &m, eval + rr_eigen_start,
+                             subspace_evec + (size_t)rr_eigen_start * ldm + rr_eigen_start, &ldm, isuppz, dwork,
+                             &lwork, ws->subspace_itmp, &liwork, ifail, &info);
+    if (info != 0) fprintf(stderr, "GCGE: dense eigensolver returned info = %d\n", info);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------
+// Convergence and multiplicity (src/gcge_eigsol.c)
+// ------------------------------------------------------------------------
+
+// GCGE_CheckConvergence, gcge_eigsol.c:559-686.  The residuals of the pairs
+// the reference visits are computed in batches: pairs up to position
+// max_conv_idx + 9 of unlock[] are always visited (the early exit needs
+// j > max_conv_idx + 8), so each batch extends exactly to that horizon and no
+// residual is computed that the reference would not compute.
+extern "C" void GCGE_CheckConvergence(void *A, void *B, GCGE_DOUBLE *eval, void **evec, GCGE_OPS *ops,
+                                      GCGE_PARA *para, GCGE_WORKSPACE *ws)
+{
+    int *unlock = ws->unlock, unconv_bs = 0, max_conv_idx = 0, max_ind = 0, min_ind = 0;
+    const int num_unlock = para->num_unlock;
+    double max_res = 0.0, min_res = 0.0, sum_res = 0.0;
+    const char *ct = para->conv_type;
+    const bool ns = std::strcmp(ct, "NS") == 0;
+    std::vector<int> idx;
+    std::vector<double> lam, rn, xn, bn;
+    int j = 0;
+    bool done = false;
+    while (j < num_unlock && !done) {
+        const int horizon = std::min(num_unlock, max_conv_idx + 10);
+        const int k = horizon - j;
+        idx.assign(unlock + j, unlock + horizon);
+        lam.resize(k);
+        rn.resize(k);
+        xn.resize(k);
+        bn.resize(k);
+        for (int t = 0; t < k; t++) lam[t] = eval[idx[t]];
+        ops->MultiVecResidualNorms(A, B, evec, idx.data(), k, lam.data(), rn.data(), xn.data(), ns ? bn.data() : nullptr,
+                                   ops);
+        for (int t = 0; t < k && !done; t++, j++) {
+            const int i = idx[t];
+            const double res_norm = rn[t], evec_norm = xn[t];
+            double residual;
+            if (std::strcmp(ct, "R") == 0)
+                residual = res_norm / evec_norm / (1.0 > std::fabs(eval[i]) ? 1.0 : std::fabs(eval[i]));
+            else if (std::strcmp(ct, "O") == 0)
+                residual = res_norm / evec_norm / (para->conv_omega_norm + std::fabs(eval[i]));
+            else if (ns)
+                residual = res_norm / (std::fabs(eval[i]) * bn[t]);
+            else
+                residual = res_norm / evec_norm;
+            para->res[i] = residual;
+            sum_res += residual;
+            if (j == 0) {
+                max_res = residual;
+                min_res = residual;
+            } else {
+                if (residual > max_res) {
+                    max_res = residual;
+                    max_ind = i;
+                }
+                if (residual < min_res) {
+                    min_res = residual;
+                    min_ind = i;
+                }
+            }
+            if (residual > para->ev_tol) {
+                unlock[unconv_bs] = i;
+                unconv_bs += 1;
+                if (j > max_conv_idx + 8) {
+                    for (int kk = j + 1; kk < num_unlock; kk++) {
+                        unlock[unconv_bs] = unlock[kk];
+                        unconv_bs += 1;
+                    }
+                    done = true;
+                }
+            } else {
+                max_conv_idx = j;
+            }
+        }
+    }
+    para->num_unlock = unconv_bs;
+    ws->unconv_bs = (unconv_bs < para->block_size) ? unconv_bs : para->block_size;
+    para->max_res = max_res;
+    para->max_ind = max_ind;
+    para->min_res = min_res;
+    para->min_ind = min_ind;
+    para->sum_res = sum_res;
+    para->num_iter += 1;
+}
+
+// GCGE_CheckEvalMultiplicity, gcge_eigsol.c:696-718 (fixed 0.2 threshold).
+extern "C" void GCGE_CheckEvalMultiplicity(GCGE_INT start, GCGE_INT nev, GCGE_INT end, GCGE_INT *dim_x,
+                                           GCGE_DOUBLE *eval)
+{
+    int tmp = start;
+    if (start < nev) {
+        fprintf(stderr, "GCGE_CheckEvalMultiplicity: start < nev\n");
+        return;
+    }
+    for (int i = start; i < end; i++) {
+        if (std::fabs(std::fabs(eval[tmp] / eval[nev]) - 1) < 0.2) tmp += 1;
+        else break;
+    }
+    *dim_x = tmp;
+}
+
+// ------------------------------------------------------------------------
+// Driver: GCGE_Solve, gcge_eigsol.c:65-542
+// ------------------------------------------------------------------------
+extern "C" void GCGE_Solve(void *A, void *B, GCGE_DOUBLE *eval, void **evec, GCGE_PARA *para, GCGE_OPS *ops,
+                           GCGE_WORKSPACE *ws)
+{
+    const double t_start = now_s();
+    const int nev = para->nev, ev_max_it = para->ev_max_it;
+    void **V = ws->V;
+    GCGE_INT ms[2], me[2];
+    ws->evec = evec;
+    para->bcg_calls = 0;
+    para->bcg_iters_total = 0;
+    void *Orth_mat, *RR_mat;
+    if (std::strcmp(para->orth_type, "B")) {
+        Orth_mat = A;
+        RR_mat = B;
+    } else {
+        Orth_mat = B;
+        RR_mat = A;
+    }
+    if (!RR_mat) {
+        fprintf(stderr, "GCGE_Solve: orth_type \"A\" needs B\n");
+        return;
+    }
+
+    {
+        GCGE_TIMER(x_orth_time, para);
+        if (para->given_init_evec == 0) {
+            int num_init_evec = 0;
+            do {
+                ops->MultiVecSetRandomValue(evec, num_init_evec, nev - num_init_evec, ops);
+                if (std::strcmp(para->conv_type, "O") == 0 && num_init_evec == 0) {
+                    // omega-norm estimate of A (:135-173)
+                    double omega_F = 0.0, AO_F = 0.0, t;
+                    std::vector<double> nrm(nev), a0(nev, 0.0), sc(nev);
+                    set2(ms, 0, 0);
+                    set2(me, nev, nev);
+                    ops->MultiVecInnerProdDiag(evec, evec, nrm.data(), ms, me, ops);
+                    for (int i = 0; i < nev; i++) sc[i] = 1.0 / std::sqrt(nrm[i]);
+                    ops->MultiVecAxpbyDiag(a0.data(), evec, sc.data(), evec, ms, me, ops);
+                    ops->MultiVecInnerProdDiag(evec, evec, nrm.data(), ms, me, ops);
+                    for (int i = 0; i < nev; i++) omega_F += nrm[i];
+                    ops->MatDotMultiVec(A, evec, V, ms, me, ops);
+                    ops->MultiVecInnerProdDiag(V, V, nrm.data(), ms, me, ops);
+                    for (int i = 0; i < nev; i++) {
+                        t = nrm[i];
+                        AO_F += t;
+                    }
+                    para->conv_omega_norm = std::sqrt(AO_F / omega_F);
+                }
+                if (para->dirichlet_boundary == 1 && ops->SetDirichletBoundary)
+                    ops->SetDirichletBoundary(evec, nev, A, B);
+                ws->dim_x = nev;
+                orth_gs(evec, num_init_evec, &ws->dim_x, Orth_mat, ops, para, ws->V_tmp, ws->subspace_dtmp);
+                num_init_evec = ws->dim_x;
+            } while (num_init_evec < nev);
+        } else {
+            orth_gs(evec, 0, &ws->dim_x, Orth_mat, ops, para, ws->V_tmp, ws->subspace_dtmp);
+        }
+    }
+    GCGE_PrintParaInfo(para);
+
+    set2(ms, 0, 0);
+    set2(me, nev, nev);
+    ops->MultiVecCopy(evec, V, ms, me, ops);
+    ws->dim_xpw = ws->dim_x;
+    ws->last_dim_x = ws->dim_x;
+    ws->dim_xp = 0;
+    rr_matrix(RR_mat, V, ops, para, ws);
+    rr_eigenpairs(ws->subspace_matrix, ws->eval, ws->subspace_evec, ops, para, ws);
+    {
+        GCGE_TIMER(x_axpy_time, para);
+        compute_x(V, ws->subspace_evec, evec, nev, ws->dim_xpw, ops, ws);
+    }
+    {
+        GCGE_TIMER(conv_time, para);
+        GCGE_CheckConvergence(A, B, ws->eval, evec, ops, para, ws);
+    }
+    if (para->print_result || para->print_eval) GCGE_PrintIterationInfo(ws->eval, para);
+
+    set2(ms, ws->num_soft_locked_in_last_iter, ws->num_soft_locked_in_last_iter);
+    set2(me, nev, nev);
+    ops->MultiVecCopy(evec, V, ms, me, ops);
+    ws->dim_xp = ws->dim_x;
+    {
+        GCGE_TIMER(w_line_time, para);
+        compute_w(A, B, V, ws->eval, ops, para, ws);
+    }
+    ws->last_dim_xpw = ws->dim_xpw;
+    ws->dim_xpw = ws->dim_xp + ws->unconv_bs;
+    {
+        GCGE_TIMER(w_orth_time, para);
+        orth_gs(V, ws->dim_x, &ws->dim_xpw, Orth_mat, ops, para, ws->V_tmp, ws->subspace_dtmp);
+    }
+    ws->dim_xp = nev;
+    rr_matrix(RR_mat, V, ops, para, ws);
+    rr_eigenpairs(ws->subspace_matrix, ws->eval, ws->subspace_evec, ops, para, ws);
+    {
+        GCGE_TIMER(x_axpy_time, para);
+        compute_x(V, ws->subspace_evec, evec, nev, ws->dim_xpw, ops, ws);
+    }
+    {
+        GCGE_TIMER(conv_time, para);
+        GCGE_CheckConvergence(A, B, ws->eval, evec, ops, para, ws);
+    }
+    if (para->print_result || para->print_eval) GCGE_PrintIterationInfo(ws->eval, para);
+
+    double best_res = para->max_res;
+    int best_iter = para->num_iter;
+    para->stopped_on_stagnation = 0;
+    while (para->num_unlock > 0 && para->num_iter < ev_max_it) {
+        if (para->stagnation_window > 0) {
+            if (para->max_res < para->stagnation_ratio * best_res) {
+                best_res = para->max_res;
+                best_iter = para->num_iter;
+            } else if (para->num_iter - best_iter >= para->stagnation_window) {
+                para->stopped_on_stagnation = 1;
+                break;
+            }
+        }
+        compute_p(ws->subspace_evec, V, ops, para, ws);
+        {
+            GCGE_TIMER(x_axpy_time, para);
+            set2(ms, ws->num_soft_locked_in_last_iter, ws->num_soft_locked_in_last_iter);
+            set2(me, nev, nev);
+            ops->MultiVecCopy(evec, V, ms, me, ops);
+        }
+        {
+            GCGE_TIMER(w_line_time, para);
+            compute_w(A, B, V, ws->eval, ops, para, ws);
+        }
+        ws->last_dim_xpw = ws->dim_xpw;
+        ws->dim_xpw = ws->dim_xp + ws->unconv_bs;
+        {
+            GCGE_TIMER(w_orth_time, para);
+            if (std::strcmp(para->w_orth_type, "multi") == 0)
+                stable_multi_orth(V, ws->dim_xp, &ws->dim_xpw, Orth_mat, ops, para, ws);
+            else
+                orth_gs(V, ws->dim_xp, &ws->dim_xpw, Orth_mat, ops, para, ws->V_tmp, ws->subspace_dtmp);
+        }
+        rr_matrix(RR_mat, V, ops, para, ws);
+        rr_eigenpairs(ws->subspace_matrix, ws->eval, ws->subspace_evec, ops, para, ws);
+        ws->last_dim_x = ws->dim_x;
+        GCGE_CheckEvalMultiplicity(nev, nev, ws->max_dim_x, &ws->dim_x, ws->eval);
+        {
+            GCGE_TIMER(x_axpy_time, para);
+            compute_x(V, ws->subspace_evec, evec, nev, ws->dim_xpw, ops, ws);
+        }
+        {
+            GCGE_TIMER(conv_time, para);
+            GCGE_CheckConvergence(A, B, ws->eval, evec, ops, para, ws);
+        }
+        if (para->print_result || para->print_eval) GCGE_PrintIterationInfo(ws->eval, para);
+    }
+
+    std::memcpy(eval, ws->eval, (size_t)nev * sizeof(double));
+    set2(ms, 0, 0);
+    set2(me, ws->num_soft_locked_in_last_iter, ws->num_soft_locked_in_last_iter);
+    if (me[0] > 0) ops->MultiVecCopy(V, evec, ms, me, ops);
+    para->solve_seconds = now_s() - t_start;
+    GCGE_PrintFinalInfo(eval, para);
+}
+
+// ------------------------------------------------------------------------
+// Workspace, ops table and solver object
+// ------------------------------------------------------------------------
+
+extern "C" void GCGE_WORKSPACE_Create(GCGE_WORKSPACE **workspace)
+{
+    *workspace = (GCGE_WORKSPACE *)calloc(1, sizeof(GCGE_WORKSPACE));
+}
+
+// GCGE_WORKSPACE_Setup, gcge_workspace.c:61-119 (same sizes).
+extern "C" void GCGE_WORKSPACE_Setup(GCGE_WORKSPACE *ws, GCGE_PARA *para, GCGE_OPS *ops, void *A)
+{
+    const int nev = para->nev, bs = para->block_size;
+    const int max_dim_x_tmp = (nev * 1.25 < nev + 8) ? (int)(nev * 1.25) : (nev + 8);
+    const int max_dim_x = (max_dim_x_tmp > nev + 3) ? max_dim_x_tmp : (nev + 3);
+    ws->V_size = max_dim_x + 2 * bs;
+    ws->V_tmp_size = max_dim_x - nev + bs;
+    ws->CG_p_size = bs;
+    ops->MultiVecCreateByMat(&ws->V, ws->V_size, A, ops);
+    ops->MultiVecCreateByMat(&ws->V_tmp, ws->V_tmp_size, A, ops);
+    ops->MultiVecCreateByMat(&ws->CG_p, ws->CG_p_size, A, ops);
+    ws->max_dim_x = max_dim_x;
+    ws->dim_x = nev;
+    ws->dim_xp = 0;
+    ws->dim_xpw = nev;
+    ws->conv_bs = 0;
+    ws->num_soft_locked_in_last_iter = 0;
+    ws->unconv_bs = (nev < bs) ? nev : bs;
+    const int max_dim_xpw = max_dim_x + 2 * bs;
+    ws->eval = (double *)calloc(max_dim_xpw, sizeof(double));
+    ws->subspace_dtmp = (double *)calloc((size_t)max_dim_xpw * max_dim_xpw + 40 * (size_t)max_dim_xpw, sizeof(double));
+    ws->subspace_itmp = (int *)calloc(100 * (size_t)max_dim_xpw, sizeof(int));
+    ws->subspace_evec = (double *)calloc((size_t)max_dim_xpw * max_dim_xpw, sizeof(double));
+    ws->subspace_matrix = (double *)calloc((size_t)max_dim_xpw * max_dim_xpw, sizeof(double));
+    ws->unlock = (int *)calloc(max_dim_x, sizeof(int));
+    for (int i = 0; i < max_dim_x; i++) ws->unlock[i] = i;
+}
+
+extern "C" void GCGE_WORKSPACE_Free(GCGE_WORKSPACE **workspace, GCGE_PARA *, GCGE_OPS *ops)
+{
+    if (!workspace || !*workspace) return;
+    GCGE_WORKSPACE *ws = *workspace;
+    free(ws->unlock);
+    free(ws->subspace_matrix);
+    free(ws->subspace_evec);
+    free(ws->eval);
+    free(ws->subspace_dtmp);
+    free(ws->subspace_itmp);
+    if (ws->V) {
+        ops->MultiVecDestroy(&ws->V, ws->V_size, ops);
+        ops->MultiVecDestroy(&ws->V_tmp, ws->V_tmp_size, ops);
+        ops->MultiVecDestroy(&ws->CG_p, ws->CG_p_size, ops);
+    }
+    free(ws);
+    *workspace = nullptr;
+}
+
+extern "C" void GCGE_OPS_Create(GCGE_OPS **ops) { *ops = (GCGE_OPS *)calloc(1, sizeof(GCGE_OPS)); }
+
+extern "C" void GCGE_SetDenseOps(GCGE_OPS *ops);
+
+// GCGE_OPS_Setup, gcge_ops.c:470-605: required slots are checked; missing
+// dense slots get the LAPACK defaults.  This build has no per-column
+// multivector defaults: the block-kernel slots must be provided by the
+// backend (GCGE_HIP_SetOps fills them all).
+extern "C" GCGE_INT GCGE_OPS_Setup(GCGE_OPS *ops)
+{
+    struct {
+        const void *p;
+        const char *name;
+    } req[] = {
+        {(const void *)ops->MatDotVec, "MatDotVec"},
+        {(const void *)ops->VecAxpby, "VecAxpby"},
+        {(const void *)ops->VecInnerProd, "VecInnerProd"},
+        {(const void *)ops->VecLocalInnerProd, "VecLocalInnerProd"},
+        {(const void *)ops->VecCreateByVec, "VecCreateByVec"},
+        {(const void *)ops->VecCreateByMat, "VecCreateByMat"},
+        {(const void *)ops->VecDestroy, "VecDestroy"},
+        {(const void *)ops->VecSetRandomValue, "VecSetRandomValue"},
+        {(const void *)ops->MultiVecCreateByMat, "MultiVecCreateByMat"},
+        {(const void *)ops->MultiVecDestroy, "MultiVecDestroy"},
+        {(const void *)ops->MultiVecSetRandomValue, "MultiVecSetRandomValue"},
+        {(const void *)ops->MatDotMultiVec, "MatDotMultiVec"},
+        {(const void *)ops->MultiVecAxpby, "MultiVecAxpby"},
+        {(const void *)ops->MultiVecAxpbyColumn, "MultiVecAxpbyColumn"},
+        {(const void *)ops->MultiVecLinearComb, "MultiVecLinearComb"},
+        {(const void *)ops->MultiVecInnerProd, "MultiVecInnerProd"},
+        {(const void *)ops->MultiVecSwap, "MultiVecSwap"},
+        {(const void *)ops->GetVecFromMultiVec, "GetVecFromMultiVec"},
+        {(const void *)ops->RestoreVecForMultiVec, "RestoreVecForMultiVec"},
+        {(const void *)ops->MultiVecAxpbyDiag, "MultiVecAxpbyDiag"},
+        {(const void *)ops->MultiVecInnerProdDiag, "MultiVecInnerProdDiag"},
+        {(const void *)ops->MultiVecLinearCombAxpby, "MultiVecLinearCombAxpby"},
+        {(const void *)ops->MatDotMultiVecInnerProdDiag, "MatDotMultiVecInnerProdDiag"},
+        {(const void *)ops->BlockCGUpdate, "BlockCGUpdate"},
+        {(const void *)ops->MultiVecResidualNorms, "MultiVecResidualNorms"},
+        {(const void *)ops->MultiVecGather, "MultiVecGather"},
+        {(const void *)ops->MultiVecCopy, "MultiVecCopy"},
+    };
+    for (auto &r : req) {
+        if (!r.p) {
+            printf("ERROR: Please provide the %s operation!\n", r.name);
+            return 1;
+        }
+    }
+    if (!ops->DenseMatEigenSolver || !ops->DenseMatDotDenseMat || !ops->DenseSymMatDotDenseMat ||
+        !ops->ArrayDotArray || !ops->ArrayNorm || !ops->ArrayAXPBY || !ops->ArrayCopy || !ops->ArrayScale) {
+        GCGE_OPS tmp;
+        std::memset(&tmp, 0, sizeof(tmp));
+        GCGE_SetDenseOps(&tmp);
+        if (!ops->DenseMatEigenSolver) ops->DenseMatEigenSolver = tmp.DenseMatEigenSolver;
+        if (!ops->DenseMatDotDenseMat) ops->DenseMatDotDenseMat = tmp.DenseMatDotDenseMat;
+        if (!ops->DenseSymMatDotDenseMat) ops->DenseSymMatDotDenseMat = tmp.DenseSymMatDotDenseMat;
+        if (!ops->ArrayDotArray) ops->ArrayDotArray = tmp.ArrayDotArray;
+        if (!ops->ArrayNorm) ops->ArrayNorm = tmp.ArrayNorm;
+        if (!ops->ArrayAXPBY) ops->ArrayAXPBY = tmp.ArrayAXPBY;
+        if (!ops->ArrayCopy) ops->ArrayCopy = tmp.ArrayCopy;
+        if (!ops->ArrayScale) ops->ArrayScale = tmp.ArrayScale;
+    }
+    return 0;
+}
+
+extern "C" void GCGE_OPS_Free(GCGE_OPS **ops)
+{
+    free(*ops);
+    *ops = nullptr;
+}
+
+extern "C" void GCGE_OPS_SetLinearSolverWorkspace(GCGE_OPS *ops, void *w) { ops->linear_solver_workspace = w; }
+
+extern "C" void GCGE_SOLVER_Create(GCGE_SOLVER **solver)
+{
+    GCGE_SOLVER *s = (GCGE_SOLVER *)calloc(1, sizeof(GCGE_SOLVER));
+    GCGE_PARA_Create(&s->para);
+    GCGE_OPS_Create(&s->ops);
+    GCGE_WORKSPACE_Create(&s->workspace);
+    *solver = s;
+}
+
+extern "C" void GCGE_SOLVER_Free(GCGE_SOLVER **solver)
+{
+    if (!solver || !*solver) return;
+    GCGE_WORKSPACE_Free(&(*solver)->workspace, (*solver)->para, (*solver)->ops);
+    GCGE_PARA_Free(&(*solver)->para);
+    GCGE_OPS_Free(&(*solver)->ops);
+    free(*solver);
+    *solver = nullptr;
+}
+
+extern "C" void GCGE_SOLVER_Free_All(GCGE_SOLVER **solver)
+{
+    if (!solver || !*solver) return;
+    GCGE_SOLVER *s = *solver;
+    if (s->evec && s->ops->MultiVecDestroy) s->ops->MultiVecDestroy(&s->evec, s->para->nev, s->ops);
+    free(s->eval);
+    s->eval = nullptr;
+    GCGE_SOLVER_Free(solver);
+}
+
+// GCGE_SOLVER_Setup, gcge_solver.c:56-61.  A failed ops setup leaves the
+// workspace empty, and GCGE_SOLVER_Solve then refuses to run.
+extern "C" void GCGE_SOLVER_Setup(GCGE_SOLVER *solver)
+{
+    GCGE_PARA_Setup(solver->para);
+    if (GCGE_OPS_Setup(solver->ops) != 0) return;
+    GCGE_WORKSPACE_Setup(solver->workspace, solver->para, solver->ops, solver->A);
+}
+
+extern "C" void GCGE_SOLVER_SetMatA(GCGE_SOLVER *s, void *A) { s->A = A; }
+extern "C" void GCGE_SOLVER_SetMatB(GCGE_SOLVER *s, void *B) { s->B = B; }
+extern "C" void GCGE_SOLVER_SetEigenvalues(GCGE_SOLVER *s, GCGE_DOUBLE *eval) { s->eval = eval; }
+extern "C" void GCGE_SOLVER_SetEigenvectors(GCGE_SOLVER *s, void **evec) { s->evec = evec; }
+extern "C" void GCGE_SOLVER_SetNumEigen(GCGE_SOLVER *s, GCGE_INT nev) { GCGE_PARA_SetNumEigen(s->para, nev); }
+extern "C" void GCGE_SOLVER_SetOpsLinearSolverWorkspace(GCGE_SOLVER *s, void *w)
+{
+    GCGE_OPS_SetLinearSolverWorkspace(s->ops, w);
+}
+extern "C" void GCGE_SOLVER_GetEigenvalues(GCGE_SOLVER *s, GCGE_DOUBLE **eval) { *eval = s->eval; }
+extern "C" void GCGE_SOLVER_GetEigenvectors(GCGE_SOLVER *s, void ***evec) { *evec = s->evec; }
+
+extern "C" void GCGE_SOLVER_Solve(GCGE_SOLVER *s)
+{
+    if (!s->workspace || !s->workspace->V) {
+        fprintf(stderr, "GCGE_SOLVER_Solve: solver is not set up (missing operations?)\n");
+        return;
+    }
+    GCGE_Solve(s->A, s->B, s->eval, s->evec, s->para, s->ops, s->workspace);
+}

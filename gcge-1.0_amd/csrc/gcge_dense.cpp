@@ -1,0 +1,162 @@
+// gcge_dense.cpp -- dense FP64 slots of GCGE_OPS (host LAPACK/BLAS).
+//
+// The subspace problems are at most (max_dim_x + 2 block_size)^2 doubles
+// (308 x 308 for 100 eigenpairs), so they stay on the host in FP64 and call
+// the same routines as the reference's defaults (src/gcge_ops.c:302-429):
+// dsyevx with range "I", dgemm, dsymm and BLAS-1.
+#include <dlfcn.h>
+
+#include <cstdio>
+#include <cstdlib>
+
+#include "gcge.h"
+
+// LAPACK/BLAS are resolved at run time from the library named by
+// GCGE_LAPACK_LIB (default: MKL's single dynamic library), so that no
+// foreign directory enters the loader search path of the HIP runtime.
+namespace {
+
+typedef void (*syevx_t)(const char *, const char *, const char *, int *, double *, int *, double *, double *, int *,
+                        int *, double *, int *, double *, double *, int *, double *, int *, int *, int *, int *);
+typedef void (*gemm_t)(const char *, const char *, int *, int *, int *, double *, double *, int *, double *, int *,
+                       double *, double *, int *);
+typedef void (*symm_t)(const char *, const char *, int *, int *, double *, double *, int *, double *, int *, double *,
+                       double *, int *);
+typedef double (*dot_t)(int *, double *, int *, double *, int *);
+typedef double (*nrm2_t)(int *, double *, int *);
+typedef void (*scal_t)(int *, double *, double *, int *);
+typedef void (*axpy_t)(int *, double *, double *, int *, double *, int *);
+typedef void (*copy_t)(int *, double *, int *, double *, int *);
+
+struct Lapack {
+    bool ok = false;
+    syevx_t dsyevx_ = nullptr;
+    gemm_t dgemm_ = nullptr;
+    symm_t dsymm_ = nullptr;
+    dot_t ddot_ = nullptr;
+    nrm2_t dnrm2_ = nullptr;
+    scal_t dscal_ = nullptr;
+    axpy_t daxpy_ = nullptr;
+    copy_t dcopy_ = nullptr;
+};
+
+Lapack &lapack()
+{
+    static Lapack L;
+    static bool tried = false;
+    if (tried) return L;
+    tried = true;
+    const char *name = getenv("GCGE_LAPACK_LIB");
+    if (!name || !*name) name = "/opt/conda/lib/libmkl_rt.so.1";
+    void *h = dlopen(name, RTLD_NOW | RTLD_LOCAL);
+    if (!h) {
+        fprintf(stderr, "GCGE: cannot load LAPACK library %s: %s\n", name, dlerror());
+        return L;
+    }
+    L.dsyevx_ = (syevx_t)dlsym(h, "dsyevx_");
+    L.dgemm_ = (gemm_t)dlsym(h, "dgemm_");
+    L.dsymm_ = (symm_t)dlsym(h, "dsymm_");
+    L.ddot_ = (dot_t)dlsym(h, "ddot_");
+    L.dnrm2_ = (nrm2_t)dlsym(h, "dnrm2_");
+    L.dscal_ = (scal_t)dlsym(h, "dscal_");
+    L.daxpy_ = (axpy_t)dlsym(h, "daxpy_");
+    L.dcopy_ = (copy_t)dlsym(h, "dcopy_");
+    L.ok = L.dsyevx_ && L.dgemm_ && L.dsymm_ && L.ddot_ && L.dnrm2_ && L.dscal_ && L.daxpy_ && L.dcopy_;
+    if (!L.ok) fprintf(stderr, "GCGE: LAPACK library %s lacks a required routine\n", name);
+    return L;
+}
+
+#define LAPACK_CALL(fn, ...)                                                     \
+    do {                                                                         \
+        Lapack &L_ = lapack();                                                   \
+        if (!L_.ok) {                                                            \
+            fprintf(stderr, "GCGE: no LAPACK available for " #fn "\n");           \
+            abort();                                                             \
+        }                                                                        \
+        L_.fn(__VA_ARGS__);                                                      \
+    } while (0)
+
+double lapack_dot(int *n, double *x, int *incx, double *y, int *incy)
+{
+    Lapack &L = lapack();
+    if (!L.ok) abort();
+    return L.ddot_(n, x, incx, y, incy);
+}
+
+double lapack_nrm2(int *n, double *x, int *incx)
+{
+    Lapack &L = lapack();
+    if (!L.ok) abort();
+    return L.dnrm2_(n, x, incx);
+}
+
+}  // namespace
+
+namespace {
+
+void d_eig(char *jobz, char *range, char *uplo, GCGE_INT *nrows, GCGE_DOUBLE *a, GCGE_INT *lda, GCGE_DOUBLE *vl,
+           GCGE_DOUBLE *vu, GCGE_INT *il, GCGE_INT *iu, GCGE_DOUBLE *abstol, GCGE_INT *nev, GCGE_DOUBLE *eval,
+           GCGE_DOUBLE *evec, GCGE_INT *lde, GCGE_INT *, GCGE_DOUBLE *work, GCGE_INT *lwork, GCGE_INT *iwork,
+           GCGE_INT *, GCGE_INT *ifail, GCGE_INT *info)
+{
+    LAPACK_CALL(dsyevx_, jobz, range, uplo, nrows, a, lda, vl, vu, il, iu, abstol, nev, eval, evec, lde, work, lwork,
+                iwork, ifail, info);
+}
+
+void d_gemm(char *ta, char *tb, GCGE_INT *m, GCGE_INT *n, GCGE_INT *k, GCGE_DOUBLE *alpha, GCGE_DOUBLE *a,
+            GCGE_INT *lda, GCGE_DOUBLE *b, GCGE_INT *ldb, GCGE_DOUBLE *beta, GCGE_DOUBLE *c, GCGE_INT *ldc)
+{
+    LAPACK_CALL(dgemm_, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+}
+
+void d_symm(char *side, char *uplo, GCGE_INT *m, GCGE_INT *n, GCGE_DOUBLE *alpha, GCGE_DOUBLE *a, GCGE_INT *lda,
+            GCGE_DOUBLE *b, GCGE_INT *ldb, GCGE_DOUBLE *beta, GCGE_DOUBLE *c, GCGE_INT *ldc)
+{
+    LAPACK_CALL(dsymm_, side, uplo, m, n, alpha, a, lda, b, ldb, beta, c, ldc);
+}
+
+GCGE_DOUBLE d_dot(GCGE_DOUBLE *x, GCGE_DOUBLE *y, GCGE_INT n)
+{
+    int inc = 1;
+    return lapack_dot(&n, x, &inc, y, &inc);
+}
+
+GCGE_DOUBLE d_nrm2(GCGE_DOUBLE *x, GCGE_INT n)
+{
+    int inc = 1;
+    return lapack_nrm2(&n, x, &inc);
+}
+
+// y = a x + b y as GCGE_Default_ArrayAXPBY (gcge_ops.c:391-405)
+void d_axpby(GCGE_DOUBLE a, GCGE_DOUBLE *x, GCGE_DOUBLE b, GCGE_DOUBLE *y, GCGE_INT n)
+{
+    int inc = 1;
+    if (b != 1.0) LAPACK_CALL(dscal_, &n, &b, y, &inc);
+    LAPACK_CALL(daxpy_, &n, &a, x, &inc, y, &inc);
+}
+
+void d_copy(GCGE_DOUBLE *x, GCGE_DOUBLE *y, GCGE_INT n)
+{
+    int inc = 1;
+    LAPACK_CALL(dcopy_, &n, x, &inc, y, &inc);
+}
+
+void d_scale(GCGE_DOUBLE a, GCGE_DOUBLE *x, GCGE_INT n)
+{
+    int inc = 1;
+    LAPACK_CALL(dscal_, &n, &a, x, &inc);
+}
+
+}  // namespace
+
+extern "C" void GCGE_SetDenseOps(GCGE_OPS *ops)
+{
+    ops->DenseMatEigenSolver = d_eig;
+    ops->DenseMatDotDenseMat = d_gemm;
+    ops->DenseSymMatDotDenseMat = d_symm;
+    ops->ArrayDotArray = d_dot;
+    ops->ArrayNorm = d_nrm2;
+    ops->ArrayAXPBY = d_axpby;
+    ops->ArrayCopy = d_copy;
+    ops->ArrayScale = d_scale;
+}

@@ -1,0 +1,123 @@
+// gcge_mixed.cpp -- mixed-precision solve: an FP32 phase of the GCG
+// iteration followed by FP64 refinement sweeps started from its eigenvectors.
+//
+// FP32 phase: matrix values and every multivector in FP32, so the SpMM, the
+// block CG and the Gram / linear-combination tiles move half the bytes.  The
+// dense Rayleigh-Ritz problem, the in-subspace orthogonalization and the
+// residual check stay FP64 (the residual kernel reads FP32 vectors into FP64
+// arithmetic).  FP32 storage bounds the attainable residual, so the phase
+// stops at fp32_tol or when the largest residual stops decreasing
+// (stagnation_window), whichever comes first.
+// FP64 phase: the FP32 eigenvectors are widened to FP64 and handed to the
+// same solver as the initial block (given_init_evec), which B-orthonormalizes
+// them and iterates to the caller's tolerance.
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+
+#include "gcge_app_hip.h"
+
+namespace {
+double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+}  // namespace
+
+extern "C" GCGE_INT GCGE_HIP_SOLVER_SolveMixed(GCGE_SOLVER *solver, const GCGE_HIP_MIXED_OPTS *opts,
+                                               GCGE_HIP_MIXED_INFO *info)
+{
+    char msg[256];
+    if (!solver || !solver->workspace || !solver->workspace->V || !solver->evec) {
+        fprintf(stderr, "GCGE_HIP_SOLVER_SolveMixed: solver is not set up\n");
+        return -1;
+    }
+    HIP_MAT *A = (HIP_MAT *)solver->A, *B = (HIP_MAT *)solver->B;
+    GCGE_PARA *P = solver->para;
+    const int nev = P->nev;
+    GCGE_HIP_MIXED_OPTS o;
+    o.fp32_tol = (opts && opts->fp32_tol > 0) ? opts->fp32_tol : 1e-4;
+    o.fp32_max_it = (opts && opts->fp32_max_it > 0) ? opts->fp32_max_it : P->ev_max_it;
+    o.fp64_max_it = (opts && opts->fp64_max_it > 0) ? opts->fp64_max_it : P->ev_max_it;
+    GCGE_HIP_MIXED_INFO inf;
+    std::memset(&inf, 0, sizeof(inf));
+
+    // ---- FP32 phase
+    const double t0 = now_s();
+    HIP_MatSetPrecision(A, GCGE_HIP_FP32);
+    if (B) HIP_MatSetPrecision(B, GCGE_HIP_FP32);
+    GCGE_SOLVER *s32 = nullptr;
+    GCGE_SOLVER_Create(&s32);
+    {
+        // same parameters as the caller's solver, own tolerance and budget
+        GCGE_ORTH_PARA *op32 = s32->para->orth_para;
+        GCGE_STATISTIC_PARA *st32 = s32->para->stat_para;
+        double *res32 = s32->para->res;
+        std::memcpy(s32->para, P, sizeof(GCGE_PARA));
+        std::memcpy(op32, P->orth_para, sizeof(GCGE_ORTH_PARA));
+        s32->para->orth_para = op32;
+        s32->para->stat_para = st32;
+        s32->para->res = res32;
+        // GCGE_PARA_Setup adds DBL_EPSILON again: undo the caller's addition
+        op32->orth_zero_tol -= DBL_EPSILON;
+        op32->scbgs_reorth_tol -= DBL_EPSILON;
+        s32->para->ev_tol = o.fp32_tol;
+        s32->para->ev_max_it = o.fp32_max_it;
+        s32->para->stagnation_window = 3;
+        s32->para->stagnation_ratio = 0.7;
+        s32->para->print_para = 0;
+        s32->para->print_result = 0;
+        s32->para->print_eval = 0;
+    }
+    s32->eval = (double *)calloc(nev, sizeof(double));
+    void **evec32 = nullptr;
+    HIP_MultiVecCreate(&evec32, nev, A);
+    if (P->given_init_evec) {
+        GCGE_INT s[2] = {0, 0}, e[2] = {nev, nev};
+        solver->ops->MultiVecCopy(solver->evec, evec32, s, e, solver->ops);
+    }
+    GCGE_SOLVER_SetMatA(s32, A);
+    if (B) GCGE_SOLVER_SetMatB(s32, B);
+    GCGE_SOLVER_SetHIPOps(s32);
+    GCGE_SOLVER_SetEigenvalues(s32, s32->eval);
+    GCGE_SOLVER_SetEigenvectors(s32, evec32);
+    GCGE_SOLVER_Setup(s32);
+    GCGE_SOLVER_Solve(s32);
+    GCGE_HIP_Synchronize();
+    inf.fp32_iters = s32->para->num_iter;
+    inf.fp32_seconds = now_s() - t0;
+
+    // ---- FP64 refinement from the FP32 eigenvectors
+    HIP_MatSetPrecision(A, GCGE_HIP_FP64);
+    if (B) HIP_MatSetPrecision(B, GCGE_HIP_FP64);
+    {
+        GCGE_INT s[2] = {0, 0}, e[2] = {nev, nev};
+        solver->ops->MultiVecCopy(evec32, solver->evec, s, e, solver->ops);
+    }
+    GCGE_SOLVER_Free_All(&s32);
+    const double t1 = now_s();
+    const int given = P->given_init_evec, maxit = P->ev_max_it;
+    P->given_init_evec = 1;
+    P->ev_max_it = o.fp64_max_it;
+    P->num_iter = 0;
+    P->num_unlock = nev;
+    P->sum_res = 0.0;
+    for (int i = 0; i < nev; i++) P->res[i] = 1.0;
+    // the workspace bookkeeping restarts as after GCGE_WORKSPACE_Setup
+    GCGE_WORKSPACE *ws = solver->workspace;
+    ws->dim_x = nev;
+    ws->dim_xp = 0;
+    ws->dim_xpw = nev;
+    ws->num_soft_locked_in_last_iter = 0;
+    ws->unconv_bs = (nev < P->block_size) ? nev : P->block_size;
+    for (int i = 0; i < ws->max_dim_x; i++) ws->unlock[i] = i;
+    GCGE_SOLVER_Solve(solver);
+    GCGE_HIP_Synchronize();
+    P->given_init_evec = given;
+    P->ev_max_it = maxit;
+    inf.fp64_iters = P->num_iter;
+    inf.fp64_seconds = now_s() - t1;
+    inf.fp64_num_unlock = P->num_unlock;
+    if (info) *info = inf;
+    return GCGE_HIP_GetLastError(msg, (GCGE_INT)sizeof(msg));
+}

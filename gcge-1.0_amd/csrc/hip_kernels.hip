@@ -1,0 +1,851 @@
+// hip_kernels.hip -- device kernels of the gfx950 GCGE backend.
+//
+// Compiled with -ffp-contract=off: element-wise kernels (SpMM, axpby, CG
+// update, residual) round every product and sum separately, in the order of
+// the reference's CPU loops (CSR_MatDotVec and CSR_VecAxpby,
+// app/csr/gcge_app_csr.c:297-365), so a row of A x or an axpby is
+// bit-identical to the CPU result.  Reductions (dots, Gram) use explicit
+// fma and a fixed two-stage order: deterministic run to run, but not the
+// CPU's sequential summation order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "hip_common.h"
+
+namespace gh {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kSpmmRows = 256;   // rows per SpMM workgroup (one row per thread)
+constexpr int kSpmmCap = 2048;   // staged nonzeros per SpMM workgroup
+constexpr int kSpmmKC = 4;       // columns per SpMM register chunk
+
+struct ColScalars {
+    double a[kMaxCols];
+    double b[kMaxCols];
+    int idx[kMaxCols];
+    int use_idx;
+};
+
+// y = a x + b y with the branch rules of CSR_VecAxpby (gcge_app_csr.c:330-365)
+template <typename T>
+__device__ __forceinline__ T axpby_el(T a, T x, T b, T y)
+{
+    if (a == T(0)) return b * y;
+    if (b == T(0)) return (a == T(1)) ? x : a * x;
+    return a * x + b * y;
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// sum over a 256-thread block; result valid in thread 0
+__device__ __forceinline__ double block_sum(double v, double *sh)
+{
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+    __syncthreads();
+    return r;
+}
+
+int grid_1d(int64_t n, int cap)
+{
+    int64_t b = (n + kBlock - 1) / kBlock;
+    if (b > cap) b = cap;
+    if (b < 1) b = 1;
+    return (int)b;
+}
+
+// ---------------------------------------------------------------- axpby
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_axpby(int64_t n, const T *__restrict__ x, int64_t ldx,
+                                                  T *__restrict__ y, int64_t ldy, ColScalars s)
+{
+    const int j = blockIdx.y;
+    const int64_t xc = s.use_idx ? (int64_t)s.idx[j] : (int64_t)j;
+    const T *xj = x + xc * ldx;
+    T *yj = y + (int64_t)j * ldy;
+    const T a = (T)s.a[j], b = (T)s.b[j];
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    if (a == T(0)) {
+        for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) yj[i] = b * yj[i];
+    } else if (b == T(0)) {
+        if (a == T(1)) {
+            for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) yj[i] = xj[i];
+        } else {
+            for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) yj[i] = a * xj[i];
+        }
+    } else {
+        for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
+            yj[i] = a * xj[i] + b * yj[i];
+    }
+}
+
+// ---------------------------------------------------------------- dots
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_dots(int64_t n, const T *__restrict__ x, int64_t ldx,
+                                                 const T *__restrict__ y, int64_t ldy,
+                                                 double *__restrict__ partial)
+{
+    __shared__ double sh[4];
+    const int j = blockIdx.y;
+    const T *xj = x + (int64_t)j * ldx;
+    const T *yj = y + (int64_t)j * ldy;
+    double acc = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
+        acc = fma((double)xj[i], (double)yj[i], acc);
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) partial[(int64_t)j * gridDim.x + blockIdx.x] = s;
+}
+
+// out[j] = sum_b partial[j * nb + b], one block per j, fixed order
+__global__ __launch_bounds__(kBlock) void k_reduce_rows(const double *__restrict__ partial, int nb,
+                                                        double *__restrict__ out)
+{
+    __shared__ double sh[4];
+    const int j = blockIdx.x;
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nb; b += kBlock) acc += partial[(int64_t)j * nb + b];
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) out[j] = s;
+}
+
+// ---------------------------------------------------------------- copy / swap / rows
+template <typename TX, typename TY>
+__global__ __launch_bounds__(kBlock) void k_copy(int64_t n, const TX *__restrict__ x, int64_t ldx,
+                                                 TY *__restrict__ y, int64_t ldy)
+{
+    const int j = blockIdx.y;
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
+        y[(int64_t)j * ldy + i] = (TY)x[(int64_t)j * ldx + i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_swap(int64_t n, T *__restrict__ x, int64_t ldx,
+                                                 T *__restrict__ y, int64_t ldy)
+{
+    const int j = blockIdx.y;
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+        T t = x[(int64_t)j * ldx + i];
+        x[(int64_t)j * ldx + i] = y[(int64_t)j * ldy + i];
+        y[(int64_t)j * ldy + i] = t;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_zero_rows(int64_t n, T *__restrict__ x, int64_t ldx,
+                                                      const unsigned char *__restrict__ mask)
+{
+    const int j = blockIdx.y;
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
+        if (mask[i]) x[(int64_t)j * ldx + i] = T(0);
+}
+
+// ---------------------------------------------------------------- SpMM
+// One thread per row; the workgroup's nonzeros are staged in LDS when they
+// fit; columns are processed kSpmmKC at a time.  Each row sum starts at 0
+// and adds val*x in CSR order, as CSR_MatDotVec does.  With DOT, the
+// workgroup also writes partial[j * nblk + blk] = sum_r X(r,j) Y(r,j).
+template <typename T, bool DOT>
+__global__ __launch_bounds__(kSpmmRows) void k_spmm(int n, const int *__restrict__ rowptr,
+                                                    const int *__restrict__ col,
+                                                    const T *__restrict__ val,
+                                                    const T *__restrict__ X, int64_t ldx,
+                                                    T *__restrict__ Y, int64_t ldy, int k,
+                                                    ColScalars g, double *__restrict__ partial)
+{
+    __shared__ int s_col[kSpmmCap];
+    __shared__ T s_val[kSpmmCap];
+    __shared__ double sh[4];
+    const int r0 = blockIdx.x * kSpmmRows;
+    const int r = r0 + threadIdx.x;
+    const int rend = min(r0 + kSpmmRows, n);
+    const int e0 = rowptr[r0];
+    const int e1 = rowptr[rend];
+    const bool staged = (e1 - e0) <= kSpmmCap;
+    if (staged) {
+        for (int e = e0 + threadIdx.x; e < e1; e += kSpmmRows) {
+            s_col[e - e0] = col[e];
+            s_val[e - e0] = val[e];
+        }
+    }
+    __syncthreads();
+    const bool active = r < n;
+    int s = 0, t = 0;
+    if (active) {
+        s = rowptr[r];
+        t = rowptr[r + 1];
+    }
+    for (int c0 = 0; c0 < k; c0 += kSpmmKC) {
+        const T *xb[kSpmmKC];
+#pragma unroll
+        for (int kk = 0; kk < kSpmmKC; kk++) {
+            int c = min(c0 + kk, k - 1);
+            int64_t xc = g.use_idx ? (int64_t)g.idx[c] : (int64_t)c;
+            xb[kk] = X + xc * ldx;
+        }
+        T acc[kSpmmKC];
+#pragma unroll
+        for (int kk = 0; kk < kSpmmKC; kk++) acc[kk] = T(0);
+        if (active) {
+            if (staged) {
+                for (int e = s; e < t; e++) {
+                    const int cj = s_col[e - e0];
+                    const T v = s_val[e - e0];
+#pragma unroll
+                    for (int kk = 0; kk < kSpmmKC; kk++) acc[kk] = acc[kk] + v * xb[kk][cj];
+                }
+            } else {
+                for (int e = s; e < t; e++) {
+                    const int cj = col[e];
+                    const T v = val[e];
+#pragma unroll
+                    for (int kk = 0; kk < kSpmmKC; kk++) acc[kk] = acc[kk] + v * xb[kk][cj];
+                }
+            }
+        }
+#pragma unroll
+        for (int kk = 0; kk < kSpmmKC; kk++) {
+            const int c = c0 + kk;
+            if (c < k) {
+                if (active) Y[(int64_t)c * ldy + r] = acc[kk];
+                if (DOT) {
+                    double d = active ? (double)xb[kk][r] * (double)acc[kk] : 0.0;
+                    double bs = block_sum(d, sh);
+                    if (threadIdx.x == 0) partial[(int64_t)c * gridDim.x + blockIdx.x] = bs;
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- residual
+// r = A x - lambda B x (x when B is absent), in FP64 arithmetic whatever the
+// storage precision of x, rounded as VecAxpby(-lambda, Bx, 1.0, Ax)
+// (gcge_eigsol.c:579-588); partial sums of r.r, x.x and Bx.Bx per column.
+template <typename TX, bool HASB>
+__global__ __launch_bounds__(kSpmmRows) void k_residual(int n, const int *__restrict__ arp,
+                                                        const int *__restrict__ acol,
+                                                        const double *__restrict__ aval,
+                                                        const int *__restrict__ brp,
+                                                        const int *__restrict__ bcol,
+                                                        const double *__restrict__ bval,
+                                                        const TX *__restrict__ X, int64_t ldx, int k,
+                                                        ColScalars g, double *__restrict__ partial)
+{
+    __shared__ double sh[4];
+    const int r = blockIdx.x * kSpmmRows + threadIdx.x;
+    const bool active = r < n;
+    const int nb = gridDim.x;
+    for (int j = 0; j < k; j++) {
+        const TX *xj = X + (int64_t)g.idx[j] * ldx;
+        const double lam = g.a[j];
+        double rr = 0.0, xx = 0.0, bb = 0.0;
+        if (active) {
+            double ax = 0.0;
+            for (int e = arp[r]; e < arp[r + 1]; e++) ax = ax + aval[e] * (double)xj[acol[e]];
+            double bx;
+            if (HASB) {
+                bx = 0.0;
+                for (int e = brp[r]; e < brp[r + 1]; e++) bx = bx + bval[e] * (double)xj[bcol[e]];
+            } else {
+                bx = (double)xj[r];
+            }
+            const double res = axpby_el<double>(-lam, bx, 1.0, ax);
+            rr = res * res;
+            xx = (double)xj[r] * (double)xj[r];
+            bb = bx * bx;
+        }
+        double s1 = block_sum(rr, sh);
+        double s2 = block_sum(xx, sh);
+        double s3 = block_sum(bb, sh);
+        if (threadIdx.x == 0) {
+            partial[(int64_t)(3 * j + 0) * nb + blockIdx.x] = s1;
+            partial[(int64_t)(3 * j + 1) * nb + blockIdx.x] = s2;
+            partial[(int64_t)(3 * j + 2) * nb + blockIdx.x] = s3;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- CG update
+// x += alpha p (VecAxpby(alpha, p, 1.0, x)); r -= alpha w
+// (VecAxpby(-alpha, w, 1.0, r)); partial r.r  (gcge_cg.c:250-276)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_update(int64_t n, T *__restrict__ x, int64_t ldx,
+                                                      const T *__restrict__ p, int64_t ldp,
+                                                      const T *__restrict__ w, int64_t ldw,
+                                                      T *__restrict__ r, int64_t ldr, ColScalars s,
+                                                      double *__restrict__ partial)
+{
+    __shared__ double sh[4];
+    const int j = blockIdx.y;
+    const T al = (T)s.a[j];
+    T *xj = x + (int64_t)j * ldx;
+    const T *pj = p + (int64_t)j * ldp;
+    const T *wj = w + (int64_t)j * ldw;
+    T *rj = r + (int64_t)j * ldr;
+    double acc = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+        xj[i] = axpby_el<T>(al, pj[i], T(1), xj[i]);
+        const T rn = axpby_el<T>(-al, wj[i], T(1), rj[i]);
+        rj[i] = rn;
+        acc = fma((double)rn, (double)rn, acc);
+    }
+    const double bs = block_sum(acc, sh);
+    if (threadIdx.x == 0) partial[(int64_t)j * gridDim.x + blockIdx.x] = bs;
+}
+
+// ---------------------------------------------------------------- Gram
+// partial[chunk][j][i] = sum over the chunk's rows of V(r, i0+i) W(r, j0+j)
+constexpr int kGT = 64;   // tile edge
+constexpr int kGR = 32;   // rows per LDS stage
+
+template <typename T, typename Acc>
+__global__ __launch_bounds__(kBlock) void k_gram(int64_t n, const T *__restrict__ V, int64_t ldv,
+                                                 int p, const T *__restrict__ W, int64_t ldw, int q,
+                                                 int64_t rows_per_chunk, double *__restrict__ partial)
+{
+    __shared__ T sV[kGR][kGT + 1];
+    __shared__ T sW[kGR][kGT + 1];
+    const int i0 = blockIdx.x * kGT, j0 = blockIdx.y * kGT;
+    const int64_t rb = (int64_t)blockIdx.z * rows_per_chunk;
+    const int64_t re = min(n, rb + rows_per_chunk);
+    const int tid = threadIdx.x, ti = tid & 15, tj = tid >> 4;
+    Acc acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = Acc(0);
+    for (int64_t r0 = rb; r0 < re; r0 += kGR) {
+        for (int e = tid; e < kGR * kGT; e += kBlock) {
+            const int rr = e % kGR, cc = e / kGR;
+            const int64_t row = r0 + rr;
+            const int ci = i0 + cc, cj = j0 + cc;
+            sV[rr][cc] = (row < re && ci < p) ? V[(int64_t)ci * ldv + row] : T(0);
+            sW[rr][cc] = (row < re && cj < q) ? W[(int64_t)cj * ldw + row] : T(0);
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int rr = 0; rr < kGR; rr++) {
+            Acc av[4], bv[4];
+#pragma unroll
+            for (int a = 0; a < 4; a++) av[a] = (Acc)sV[rr][ti + 16 * a];
+#pragma unroll
+            for (int b = 0; b < 4; b++) bv[b] = (Acc)sW[rr][tj + 16 * b];
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 4; b++) acc[a][b] = fma(av[a], bv[b], acc[a][b]);
+        }
+        __syncthreads();
+    }
+    double *out = partial + (int64_t)blockIdx.z * p * q;
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const int i = i0 + ti + 16 * a, j = j0 + tj + 16 * b;
+            if (i < p && j < q) out[(int64_t)j * p + i] = (double)acc[a][b];
+        }
+}
+
+// G[e] = sum_c partial[c][e], e < pq; one thread per entry, fixed order
+__global__ __launch_bounds__(kBlock) void k_reduce_chunks(const double *__restrict__ partial,
+                                                          int nchunk, int64_t pq,
+                                                          double *__restrict__ G)
+{
+    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (e >= pq) return;
+    double acc = 0.0;
+    for (int c = 0; c < nchunk; c++) acc += partial[(int64_t)c * pq + e];
+    G[e] = acc;
+}
+
+// ---------------------------------------------------------------- LinComb
+// Y(:, j0+j) = beta Y + alpha sum_i X(:, i) C[j*p + i]; tile 128 rows x 64 cols
+constexpr int kLR = 128;
+constexpr int kLQ = 64;
+constexpr int kLK = 16;
+
+template <typename T, typename Acc>
+__global__ __launch_bounds__(kBlock) void k_lincomb(int64_t n, const T *__restrict__ X, int64_t ldx,
+                                                    int p, T *__restrict__ Y, int64_t ldy, int q,
+                                                    const double *__restrict__ C, double alpha,
+                                                    double beta)
+{
+    __shared__ T sX[kLK][kLR + 1];
+    __shared__ Acc sC[kLK][kLQ + 1];
+    const int64_t row0 = (int64_t)blockIdx.x * kLR;
+    const int j0 = blockIdx.y * kLQ;
+    const int tid = threadIdx.x, tr = tid & 15, tc = tid >> 4;
+    Acc acc[8][4];
+#pragma unroll
+    for (int a = 0; a < 8; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = Acc(0);
+    for (int k0 = 0; k0 < p; k0 += kLK) {
+        for (int e = tid; e < kLK * kLR; e += kBlock) {
+            const int rr = e % kLR, kk = e / kLR;
+            const int64_t row = row0 + rr;
+            sX[kk][rr] = (row < n && k0 + kk < p) ? X[(int64_t)(k0 + kk) * ldx + row] : T(0);
+        }
+        for (int e = tid; e < kLK * kLQ; e += kBlock) {
+            const int kk = e % kLK, cc = e / kLK;
+            sC[kk][cc] = (k0 + kk < p && j0 + cc < q) ? (Acc)C[(int64_t)(j0 + cc) * p + k0 + kk] : Acc(0);
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int kk = 0; kk < kLK; kk++) {
+            Acc xv[8], cv[4];
+#pragma unroll
+            for (int a = 0; a < 8; a++) xv[a] = (Acc)sX[kk][tr + 16 * a];
+#pragma unroll
+            for (int b = 0; b < 4; b++) cv[b] = sC[kk][tc + 16 * b];
+#pragma unroll
+            for (int a = 0; a < 8; a++)
+#pragma unroll
+                for (int b = 0; b < 4; b++) acc[a][b] = fma(xv[a], cv[b], acc[a][b]);
+        }
+        __syncthreads();
+    }
+    const T al = (T)alpha, be = (T)beta;
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+        const int j = j0 + tc + 16 * b;
+        if (j >= q) continue;
+        T *yj = Y + (int64_t)j * ldy;
+#pragma unroll
+        for (int a = 0; a < 8; a++) {
+            const int64_t row = row0 + tr + 16 * a;
+            if (row < n) {
+                const T v = (T)acc[a][b];
+                yj[row] = (be == T(0)) ? ((al == T(1)) ? v : al * v) : be * yj[row] + al * v;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- gradient
+// g_e = sum_i w_i x_i[row_e] x_i[col_e], one thread per stored entry
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sampled_outer(int nnz, const int *__restrict__ rowidx,
+                                                          const int *__restrict__ col,
+                                                          const T *__restrict__ X, int64_t ldx,
+                                                          int k, ColScalars s, double *__restrict__ g)
+{
+    const int e = blockIdx.x * kBlock + threadIdx.x;
+    if (e >= nnz) return;
+    const int r = rowidx[e], c = col[e];
+    double acc = 0.0;
+    for (int i = 0; i < k; i++) {
+        const T *xi = X + (int64_t)i * ldx;
+        acc = fma(s.a[i], (double)xi[r] * (double)xi[c], acc);
+    }
+    g[e] += acc;
+}
+
+__global__ __launch_bounds__(kBlock) void k_csr_gather_sum(int nparam, const int *__restrict__ jrp,
+                                                           const int *__restrict__ jnz,
+                                                           const double *__restrict__ jval,
+                                                           const double *__restrict__ g,
+                                                           double *__restrict__ grad)
+{
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= nparam) return;
+    double acc = 0.0;
+    for (int t = jrp[p]; t < jrp[p + 1]; t++) acc = fma(jval[t], g[jnz[t]], acc);
+    grad[p] = acc;
+}
+
+__global__ __launch_bounds__(kBlock) void k_rowidx(int n, const int *__restrict__ rowptr,
+                                                   int *__restrict__ rowidx)
+{
+    const int r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= n) return;
+    for (int e = rowptr[r]; e < rowptr[r + 1]; e++) rowidx[e] = r;
+}
+
+__global__ __launch_bounds__(kBlock) void k_to_f32(int64_t n, const double *__restrict__ a,
+                                                   float *__restrict__ b)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+
+}  // namespace
+
+// ======================================================================
+// launchers
+// ======================================================================
+
+static ColScalars make_scalars(int k, const double *a, const double *b, const int *idx)
+{
+    ColScalars s;
+    std::memset(&s, 0, sizeof(s));
+    for (int j = 0; j < k; j++) {
+        s.a[j] = a ? a[j] : 0.0;
+        s.b[j] = b ? b[j] : 0.0;
+        s.idx[j] = idx ? idx[j] : j;
+    }
+    s.use_idx = idx ? 1 : 0;
+    return s;
+}
+
+void launch_axpby(int prec, int64_t n, const void *x, int64_t ldx, void *y, int64_t ldy, int k,
+                  const double *a, const double *b, const int *xidx)
+{
+    if (k <= 0 || n <= 0 || failed()) return;
+    ColScalars s = make_scalars(k, a, b, xidx);
+    dim3 grid(grid_1d(n, 1024), k);
+    double bytes = 0.0;
+    for (int j = 0; j < k; j++) bytes += (double)n * elem_size(prec) * ((s.a[j] == 0.0 || s.b[j] == 0.0) ? 2 : 3);
+    ProfScope ps(K_AXPBY, bytes, 0.0);
+    if (prec == GCGE_HIP_FP64)
+        k_axpby<double><<<grid, kBlock, 0, ctx().stream>>>(n, (const double *)x, ldx, (double *)y, ldy, s);
+    else
+        k_axpby<float><<<grid, kBlock, 0, ctx().stream>>>(n, (const float *)x, ldx, (float *)y, ldy, s);
+    check(hipGetLastError(), "k_axpby");
+}
+
+static bool reduce_to_host(double *partial, int rows, int nb, double *out)
+{
+    double *dres = partial + (int64_t)rows * nb;
+    k_reduce_rows<<<rows, kBlock, 0, ctx().stream>>>(partial, nb, dres);
+    if (!check(hipGetLastError(), "k_reduce_rows")) return false;
+    double *h = host_buf(rows);
+    if (!h) return false;
+    if (!check(hipMemcpyAsync(h, dres, sizeof(double) * rows, hipMemcpyDeviceToHost, ctx().stream),
+               "hipMemcpyAsync D2H"))
+        return false;
+    if (!sync()) return false;
+    std::memcpy(out, h, sizeof(double) * rows);
+    return true;
+}
+
+bool dots(int prec, int64_t n, const void *x, int64_t ldx, const void *y, int64_t ldy, int k,
+          double *out)
+{
+    if (k <= 0) return true;
+    if (failed()) return false;
+    const int nb = grid_1d(n, 256);
+    double *partial = (double *)scratch(sizeof(double) * ((size_t)k * nb + k));
+    if (!partial) return false;
+    dim3 grid(nb, k);
+    {
+        ProfScope ps(K_DOTS, 2.0 * k * (double)n * elem_size(prec), 2.0 * k * (double)n);
+        if (prec == GCGE_HIP_FP64)
+            k_dots<double><<<grid, kBlock, 0, ctx().stream>>>(n, (const double *)x, ldx, (const double *)y, ldy,
+                                                             partial);
+        else
+            k_dots<float><<<grid, kBlock, 0, ctx().stream>>>(n, (const float *)x, ldx, (const float *)y, ldy,
+                                                            partial);
+    }
+    if (!check(hipGetLastError(), "k_dots")) return false;
+    return reduce_to_host(partial, k, nb, out);
+}
+
+void launch_copy(int xprec, const void *x, int64_t ldx, int yprec, void *y, int64_t ldy, int64_t n, int k)
+{
+    if (k <= 0 || n <= 0 || failed()) return;
+    dim3 grid(grid_1d(n, 1024), k);
+    hipStream_t st = ctx().stream;
+    ProfScope ps(K_COPY, (double)k * n * (elem_size(xprec) + elem_size(yprec)), 0.0);
+    if (xprec == GCGE_HIP_FP64 && yprec == GCGE_HIP_FP64)
+        k_copy<double, double><<<grid, kBlock, 0, st>>>(n, (const double *)x, ldx, (double *)y, ldy);
+    else if (xprec == GCGE_HIP_FP64)
+        k_copy<double, float><<<grid, kBlock, 0, st>>>(n, (const double *)x, ldx, (float *)y, ldy);
+    else if (yprec == GCGE_HIP_FP64)
+        k_copy<float, double><<<grid, kBlock, 0, st>>>(n, (const float *)x, ldx, (double *)y, ldy);
+    else
+        k_copy<float, float><<<grid, kBlock, 0, st>>>(n, (const float *)x, ldx, (float *)y, ldy);
+    check(hipGetLastError(), "k_copy");
+}
+
+void launch_swap(int prec, void *x, int64_t ldx, void *y, int64_t ldy, int64_t n, int k)
+{
+    if (k <= 0 || n <= 0 || failed()) return;
+    dim3 grid(grid_1d(n, 1024), k);
+    ProfScope ps(K_SWAP, 4.0 * k * (double)n * elem_size(prec), 0.0);
+    if (prec == GCGE_HIP_FP64)
+        k_swap<double><<<grid, kBlock, 0, ctx().stream>>>(n, (double *)x, ldx, (double *)y, ldy);
+    else
+        k_swap<float><<<grid, kBlock, 0, ctx().stream>>>(n, (float *)x, ldx, (float *)y, ldy);
+    check(hipGetLastError(), "k_swap");
+}
+
+void launch_zero_rows(int prec, void *x, int64_t ldx, int64_t n, int k, const unsigned char *mask)
+{
+    if (k <= 0 || n <= 0 || failed()) return;
+    dim3 grid(grid_1d(n, 1024), k);
+    if (prec == GCGE_HIP_FP64)
+        k_zero_rows<double><<<grid, kBlock, 0, ctx().stream>>>(n, (double *)x, ldx, mask);
+    else
+        k_zero_rows<float><<<grid, kBlock, 0, ctx().stream>>>(n, (float *)x, ldx, mask);
+    check(hipGetLastError(), "k_zero_rows");
+}
+
+template <typename T>
+static const T *mat_vals(HIP_MAT *A);
+template <>
+const double *mat_vals<double>(HIP_MAT *A) { return A->val64; }
+template <>
+const float *mat_vals<float>(HIP_MAT *A) { return mat_val32(A); }
+
+template <typename T, bool DOT>
+static bool spmm_t(HIP_MAT *A, const void *X, int64_t ldx, void *Y, int64_t ldy, int k, const int *xidx,
+                   double *out)
+{
+    const int nblk = (A->n + kSpmmRows - 1) / kSpmmRows;
+    const T *val = mat_vals<T>(A);
+    if (!val) return false;
+    for (int c0 = 0; c0 < k; c0 += kMaxCols) {
+        const int kc = std::min(kMaxCols, k - c0);
+        ColScalars g = make_scalars(kc, nullptr, nullptr, xidx ? xidx + c0 : nullptr);
+        const T *Xc = (const T *)X + (xidx ? 0 : (int64_t)c0 * ldx);
+        T *Yc = (T *)Y + (int64_t)c0 * ldy;
+        double *partial = nullptr;
+        if (DOT) {
+            partial = (double *)scratch(sizeof(double) * ((size_t)kc * nblk + kc));
+            if (!partial) return false;
+        }
+        {
+            const double bytes = (sizeof(T) + 4.0) * A->nnz + 4.0 * (A->n + 1) + 2.0 * kc * (double)A->n * sizeof(T);
+            ProfScope ps(DOT ? K_SPMM_DOT : K_SPMM, bytes, 2.0 * kc * (double)A->nnz);
+            k_spmm<T, DOT><<<nblk, kSpmmRows, 0, ctx().stream>>>(A->n, A->rowptr, A->col, val, Xc, ldx, Yc, ldy,
+                                                                  kc, g, partial);
+        }
+        if (!check(hipGetLastError(), "k_spmm")) return false;
+        if (DOT && !reduce_to_host(partial, kc, nblk, out + c0)) return false;
+    }
+    return true;
+}
+
+void launch_spmm(int prec, HIP_MAT *A, const void *X, int64_t ldx, void *Y, int64_t ldy, int k, const int *xidx)
+{
+    if (k <= 0 || failed()) return;
+    if (prec == GCGE_HIP_FP64) spmm_t<double, false>(A, X, ldx, Y, ldy, k, xidx, nullptr);
+    else spmm_t<float, false>(A, X, ldx, Y, ldy, k, xidx, nullptr);
+}
+
+bool spmm_dot(int prec, HIP_MAT *A, const void *X, int64_t ldx, void *Y, int64_t ldy, int k, double *out)
+{
+    if (k <= 0) return true;
+    if (failed()) return false;
+    if (prec == GCGE_HIP_FP64) return spmm_t<double, true>(A, X, ldx, Y, ldy, k, nullptr, out);
+    return spmm_t<float, true>(A, X, ldx, Y, ldy, k, nullptr, out);
+}
+
+template <typename T>
+static bool residual_t(HIP_MAT *A, HIP_MAT *B, const void *X, int64_t ldx, const int *idx, int k,
+                       const double *lambda, double *rnorm, double *xnorm, double *bxnorm)
+{
+    const int nblk = (A->n + kSpmmRows - 1) / kSpmmRows;
+    std::vector<double> tmp(3 * kMaxCols);
+    for (int c0 = 0; c0 < k; c0 += kMaxCols) {
+        const int kc = std::min(kMaxCols, k - c0);
+        ColScalars g = make_scalars(kc, lambda + c0, nullptr, idx + c0);
+        double *partial = (double *)scratch(sizeof(double) * ((size_t)3 * kc * nblk + 3 * kc));
+        if (!partial) return false;
+        const double rbytes = 12.0 * A->nnz + 4.0 * (A->n + 1) + (B ? 12.0 * B->nnz + 4.0 * (B->n + 1) : 0.0) +
+                              (double)kc * A->n * sizeof(T);
+        {
+            ProfScope ps(K_RESIDUAL, rbytes, 2.0 * kc * (double)(A->nnz + (B ? B->nnz : 0)));
+            if (B)
+                k_residual<T, true><<<nblk, kSpmmRows, 0, ctx().stream>>>(
+                    A->n, A->rowptr, A->col, A->val64, B->rowptr, B->col, B->val64, (const T *)X, ldx, kc, g,
+                    partial);
+            else
+                k_residual<T, false><<<nblk, kSpmmRows, 0, ctx().stream>>>(
+                    A->n, A->rowptr, A->col, A->val64, nullptr, nullptr, nullptr, (const T *)X, ldx, kc, g,
+                    partial);
+        }
+        if (!check(hipGetLastError(), "k_residual")) return false;
+        if (!reduce_to_host(partial, 3 * kc, nblk, tmp.data())) return false;
+        for (int j = 0; j < kc; j++) {
+            rnorm[c0 + j] = std::sqrt(tmp[3 * j + 0]);
+            xnorm[c0 + j] = std::sqrt(tmp[3 * j + 1]);
+            if (bxnorm) bxnorm[c0 + j] = std::sqrt(tmp[3 * j + 2]);
+        }
+    }
+    return true;
+}
+
+bool residual_norms(int prec, HIP_MAT *A, HIP_MAT *B, const void *X, int64_t ldx, const int *idx, int k,
+                    const double *lambda, double *rnorm, double *xnorm, double *bxnorm)
+{
+    if (k <= 0) return true;
+    if (failed()) return false;
+    if (prec == GCGE_HIP_FP64) return residual_t<double>(A, B, X, ldx, idx, k, lambda, rnorm, xnorm, bxnorm);
+    return residual_t<float>(A, B, X, ldx, idx, k, lambda, rnorm, xnorm, bxnorm);
+}
+
+bool cg_update(int prec, int64_t n, void *x, int64_t ldx, const void *p, int64_t ldp, const void *w,
+               int64_t ldw, void *r, int64_t ldr, int k, const double *alpha, double *rr)
+{
+    if (k <= 0) return true;
+    if (failed()) return false;
+    const int nb = grid_1d(n, 256);
+    for (int c0 = 0; c0 < k; c0 += kMaxCols) {
+        const int kc = std::min(kMaxCols, k - c0);
+        ColScalars s = make_scalars(kc, alpha + c0, nullptr, nullptr);
+        double *partial = (double *)scratch(sizeof(double) * ((size_t)kc * nb + kc));
+        if (!partial) return false;
+        dim3 grid(nb, kc);
+        {
+            ProfScope ps(K_CG_UPDATE, 6.0 * kc * (double)n * elem_size(prec), 6.0 * kc * (double)n);
+            if (prec == GCGE_HIP_FP64)
+                k_cg_update<double><<<grid, kBlock, 0, ctx().stream>>>(
+                    n, (double *)x + (int64_t)c0 * ldx, ldx, (const double *)p + (int64_t)c0 * ldp, ldp,
+                    (const double *)w + (int64_t)c0 * ldw, ldw, (double *)r + (int64_t)c0 * ldr, ldr, s, partial);
+            else
+                k_cg_update<float><<<grid, kBlock, 0, ctx().stream>>>(
+                    n, (float *)x + (int64_t)c0 * ldx, ldx, (const float *)p + (int64_t)c0 * ldp, ldp,
+                    (const float *)w + (int64_t)c0 * ldw, ldw, (float *)r + (int64_t)c0 * ldr, ldr, s, partial);
+        }
+        if (!check(hipGetLastError(), "k_cg_update")) return false;
+        if (!reduce_to_host(partial, kc, nb, rr + c0)) return false;
+    }
+    return true;
+}
+
+bool gram(int prec, int64_t n, const void *V, int64_t ldv, int p, const void *W, int64_t ldw, int q,
+          double *G, int ldg, bool upper_only)
+{
+    (void)upper_only;
+    if (p <= 0 || q <= 0) return true;
+    if (failed()) return false;
+    const int pt = (p + kGT - 1) / kGT, qt = (q + kGT - 1) / kGT;
+    int nchunk = (int)std::min<int64_t>(kMaxChunks, std::max<int64_t>(1, (2048 + pt * qt - 1) / (pt * qt)));
+    int64_t rpc = (n + nchunk - 1) / nchunk;
+    rpc = ((rpc + kGR - 1) / kGR) * kGR;
+    if (rpc < kGR) rpc = kGR;
+    nchunk = (int)((n + rpc - 1) / rpc);
+    if (nchunk < 1) nchunk = 1;
+    const int64_t pq = (int64_t)p * q;
+    double *partial = (double *)scratch(sizeof(double) * ((size_t)nchunk * pq + pq));
+    if (!partial) return false;
+    double *dG = partial + (int64_t)nchunk * pq;
+    dim3 grid(pt, qt, nchunk);
+    {
+        ProfScope ps(K_GRAM, (double)n * (p + q) * elem_size(prec) + 8.0 * pq, 2.0 * (double)n * pq);
+        if (prec == GCGE_HIP_FP64)
+            k_gram<double, double><<<grid, kBlock, 0, ctx().stream>>>(n, (const double *)V, ldv, p,
+                                                                      (const double *)W, ldw, q, rpc, partial);
+        else
+            k_gram<float, float><<<grid, kBlock, 0, ctx().stream>>>(n, (const float *)V, ldv, p,
+                                                                    (const float *)W, ldw, q, rpc, partial);
+    }
+    if (!check(hipGetLastError(), "k_gram")) return false;
+    k_reduce_chunks<<<(int)((pq + kBlock - 1) / kBlock), kBlock, 0, ctx().stream>>>(partial, nchunk, pq, dG);
+    if (!check(hipGetLastError(), "k_reduce_chunks")) return false;
+    double *h = host_buf((size_t)pq);
+    if (!h) return false;
+    if (!check(hipMemcpyAsync(h, dG, sizeof(double) * pq, hipMemcpyDeviceToHost, ctx().stream), "D2H gram"))
+        return false;
+    if (!sync()) return false;
+    for (int j = 0; j < q; j++)
+        for (int i = 0; i < p; i++) G[(int64_t)j * ldg + i] = h[(int64_t)j * p + i];
+    return true;
+}
+
+bool lincomb(int prec, int64_t n, const void *X, int64_t ldx, int p, void *Y, int64_t ldy, int q,
+             const double *C, int ldc, double alpha, double beta)
+{
+    if (q <= 0 || n <= 0) return true;
+    if (failed()) return false;
+    if (p <= 0) {
+        // empty sum: Y = beta Y
+        std::vector<double> b(kMaxCols, beta), a(kMaxCols, 0.0);
+        for (int c0 = 0; c0 < q; c0 += kMaxCols)
+            launch_axpby(prec, n, Y, ldy, (char *)Y + (size_t)c0 * ldy * elem_size(prec), ldy,
+                         std::min(kMaxCols, q - c0), a.data(), b.data(), nullptr);
+        return !failed();
+    }
+    // pack C (p x q) contiguously and upload
+    std::vector<double> hc((size_t)p * q);
+    for (int j = 0; j < q; j++)
+        for (int i = 0; i < p; i++) hc[(size_t)j * p + i] = C[(int64_t)j * ldc + i];
+    double *dC = (double *)scratch(sizeof(double) * hc.size());
+    if (!dC) return false;
+    if (!check(hipMemcpyAsync(dC, hc.data(), sizeof(double) * hc.size(), hipMemcpyHostToDevice, ctx().stream),
+               "H2D lincomb coef"))
+        return false;
+    dim3 grid((unsigned)((n + kLR - 1) / kLR), (q + kLQ - 1) / kLQ);
+    {
+        ProfScope ps(K_LINCOMB, (double)n * (p + q + (beta != 0.0 ? q : 0)) * elem_size(prec),
+                     2.0 * (double)n * p * q);
+        if (prec == GCGE_HIP_FP64)
+            k_lincomb<double, double><<<grid, kBlock, 0, ctx().stream>>>(n, (const double *)X, ldx, p,
+                                                                         (double *)Y, ldy, q, dC, alpha, beta);
+        else
+            k_lincomb<float, float><<<grid, kBlock, 0, ctx().stream>>>(n, (const float *)X, ldx, p, (float *)Y,
+                                                                       ldy, q, dC, alpha, beta);
+    }
+    if (!check(hipGetLastError(), "k_lincomb")) return false;
+    // the host staging vector dies here: make sure the copy has completed
+    return sync();
+}
+
+bool sampled_outer(int prec, HIP_MAT *A, const void *X, int64_t ldx, int k, const double *w, double *g_dev)
+{
+    if (failed()) return false;
+    const int *rowidx = mat_rowidx(A);
+    if (!rowidx) return false;
+    if (!check(hipMemsetAsync(g_dev, 0, sizeof(double) * A->nnz, ctx().stream), "memset g")) return false;
+    const int nb = (A->nnz + kBlock - 1) / kBlock;
+    for (int c0 = 0; c0 < k; c0 += kMaxCols) {
+        const int kc = std::min(kMaxCols, k - c0);
+        ColScalars s = make_scalars(kc, w + c0, nullptr, nullptr);
+        if (prec == GCGE_HIP_FP64)
+            k_sampled_outer<double><<<nb, kBlock, 0, ctx().stream>>>(A->nnz, rowidx, A->col,
+                                                                     (const double *)X + (int64_t)c0 * ldx, ldx,
+                                                                     kc, s, g_dev);
+        else
+            k_sampled_outer<float><<<nb, kBlock, 0, ctx().stream>>>(A->nnz, rowidx, A->col,
+                                                                    (const float *)X + (int64_t)c0 * ldx, ldx,
+                                                                    kc, s, g_dev);
+        if (!check(hipGetLastError(), "k_sampled_outer")) return false;
+    }
+    return true;
+}
+
+void launch_csr_gather_sum(int nparam, const int *jrowptr, const int *jnz, const double *jval, const double *g,
+                           double *grad)
+{
+    if (nparam <= 0 || failed()) return;
+    k_csr_gather_sum<<<(nparam + kBlock - 1) / kBlock, kBlock, 0, ctx().stream>>>(nparam, jrowptr, jnz, jval, g,
+                                                                                   grad);
+    check(hipGetLastError(), "k_csr_gather_sum");
+}
+
+// matrix helpers used by hip_backend.cpp
+void launch_rowidx(HIP_MAT *A)
+{
+    k_rowidx<<<(A->n + kBlock - 1) / kBlock, kBlock, 0, ctx().stream>>>(A->n, A->rowptr, A->rowidx);
+    check(hipGetLastError(), "k_rowidx");
+}
+
+void launch_to_f32(int64_t n, const double *a, float *b)
+{
+    k_to_f32<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, ctx().stream>>>(n, a, b);
+    check(hipGetLastError(), "k_to_f32");
+}
+
+}  // namespace gh

@@ -1,0 +1,344 @@
+"""High-level Python access to the gfx950 GCGE backend, through its C ABI.
+
+``solve`` drives the same calls a C program would make: HIP_MatCreateFromCSR,
+GCGE_HIP_Solver_Init (options as -gcge_* arguments), HIP_MultiVecSetFromHost
+for the initial block, GCGE_SOLVER_Solve or GCGE_HIP_SOLVER_SolveMixed, then
+reads eval/evec back.  ``Ops`` calls individual GCGE_OPS slots (the device
+kernels) on device multivectors, for kernel-level tests and measurements.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+
+import numpy as np
+
+from . import capi
+from .capi import DP, IP
+
+# options parsed by GCGE_PARA_SetFromCommandLine (set before the workspace is sized)
+_CLI = {"ev_max_it", "block_size", "given_init_evec", "ev_tol", "conv_type", "conv_omega_norm", "orth_type",
+        "w_orth_type", "p_orth_type", "x_orth_type", "orth_zero_tol", "reorth_tol", "scbgs_reorth_tol",
+        "max_reorth_time", "max_direct_orth_length", "multi_tol", "multi_tol_for_lock", "if_use_bcg",
+        "cg_max_it", "cg_rate", "cg_type", "print_eval", "print_part_time", "print_para", "print_result",
+        "opt_rr_eig_partly", "x_orth_block_size", "w_orth_block_size", "if_use_cg", "print_cg_error",
+        "print_orth_zero", "scbgs_wself_max_reorth_time"}
+# members set directly after initialisation (read at solve time)
+_DIRECT = {"dirichlet_boundary", "stagnation_window", "stagnation_ratio"}
+
+
+def _dp(a):
+    return a.ctypes.data_as(DP)
+
+
+def _ip(a):
+    return a.ctypes.data_as(IP)
+
+
+def _check(what):
+    capi.raise_on_error(what)
+
+
+class Matrix:
+    """A CSR matrix uploaded to the device (HIP_MAT)."""
+
+    def __init__(self, M):
+        L = capi.load()
+        self.n = M.shape[0]
+        self.rowptr = np.ascontiguousarray(M.indptr, dtype=np.int32)
+        self.col = np.ascontiguousarray(M.indices, dtype=np.int32)
+        self.val = np.ascontiguousarray(M.data, dtype=np.float64)
+        self.nnz = int(self.col.size)
+        self.ptr = L.HIP_MatCreateFromCSR(self.n, M.shape[1], self.nnz, _ip(self.rowptr), _ip(self.col),
+                                          _dp(self.val))
+        if not self.ptr:
+            _check("HIP_MatCreateFromCSR")
+            raise RuntimeError("HIP_MatCreateFromCSR failed")
+
+    def set_values(self, val):
+        v = np.ascontiguousarray(val, dtype=np.float64)
+        capi.load().HIP_MatSetValues(self.ptr, _dp(v))
+        _check("HIP_MatSetValues")
+
+    def set_precision(self, prec):
+        capi.load().HIP_MatSetPrecision(self.ptr, prec)
+        _check("HIP_MatSetPrecision")
+
+    def close(self):
+        if self.ptr:
+            p = C.c_void_p(self.ptr)
+            capi.load().HIP_MatDestroy(C.byref(p))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class MultiVec:
+    """A device multivector (n x ncols, column-major) created from a Matrix."""
+
+    def __init__(self, mat: Matrix, ncols: int, prec: int = capi.GCGE_HIP_FP64, data=None):
+        L = capi.load()
+        mat.set_precision(prec)
+        h = C.c_void_p()
+        L.HIP_MultiVecCreate(C.byref(h), ncols, mat.ptr)
+        mat.set_precision(capi.GCGE_HIP_FP64)
+        _check("HIP_MultiVecCreate")
+        self.ptr = h.value
+        self.n = mat.n
+        self.ncols = ncols
+        self.prec = prec
+        if data is not None:
+            self.set(data)
+
+    def set(self, X, start=0):
+        X = np.asfortranarray(X, dtype=np.float64)
+        if X.ndim == 1:
+            X = X[:, None]
+        capi.load().HIP_MultiVecSetFromHost(self.ptr, start, X.shape[1], _dp(X), X.shape[0])
+        _check("HIP_MultiVecSetFromHost")
+
+    def get(self, start=0, ncols=None):
+        ncols = self.ncols - start if ncols is None else ncols
+        X = np.zeros((self.n, ncols), order="F")
+        capi.load().HIP_MultiVecGetToHost(self.ptr, start, ncols, _dp(X), self.n)
+        _check("HIP_MultiVecGetToHost")
+        return X
+
+    def close(self):
+        if self.ptr:
+            p = C.c_void_p(self.ptr)
+            capi.load().HIP_MultiVecDestroy(C.byref(p))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _r(a, b):
+    return (C.c_int * 2)(*a), (C.c_int * 2)(*b)
+
+
+class Ops:
+    """Calls GCGE_OPS slots of the HIP backend on device multivectors."""
+
+    def __init__(self):
+        L = capi.load()
+        self._p = C.POINTER(capi.GCGE_OPS)()
+        L.GCGE_OPS_Create(C.byref(self._p))
+        L.GCGE_HIP_SetOps(self._p)
+        self.o = self._p.contents
+
+    def _f(self, name, ftype):
+        return C.cast(getattr(self.o, name), ftype)
+
+    def matdot(self, A: Matrix, X: MultiVec, Y: MultiVec, xs=0, ys=0, k=None):
+        k = X.ncols - xs if k is None else k
+        s, e = _r((xs, ys), (xs + k, ys + k))
+        self._f("MatDotMultiVec", capi.F_MatDotMultiVec)(A.ptr, X.ptr, Y.ptr, s, e, self._p)
+        _check("MatDotMultiVec")
+
+    def matdot_dot(self, A: Matrix, X: MultiVec, Y: MultiVec, k=None):
+        k = X.ncols if k is None else k
+        out = np.zeros(k)
+        s, e = _r((0, 0), (k, k))
+        self._f("MatDotMultiVecInnerProdDiag", capi.F_MatDotMultiVecInnerProdDiag)(A.ptr, X.ptr, Y.ptr, s, e,
+                                                                                    _dp(out), self._p)
+        _check("MatDotMultiVecInnerProdDiag")
+        return out
+
+    def axpby(self, a, X: MultiVec, b, Y: MultiVec, xs=0, ys=0, k=None):
+        k = X.ncols - xs if k is None else k
+        s, e = _r((xs, ys), (xs + k, ys + k))
+        self._f("MultiVecAxpby", capi.F_MultiVecAxpby)(a, X.ptr, b, Y.ptr, s, e, self._p)
+        _check("MultiVecAxpby")
+
+    def axpby_diag(self, a, X: MultiVec, b, Y: MultiVec):
+        k = len(a)
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        s, e = _r((0, 0), (k, k))
+        self._f("MultiVecAxpbyDiag", capi.F_MultiVecAxpbyDiag)(_dp(a), X.ptr, _dp(b), Y.ptr, s, e, self._p)
+        _check("MultiVecAxpbyDiag")
+
+    def dots(self, X: MultiVec, Y: MultiVec, k=None):
+        k = X.ncols if k is None else k
+        out = np.zeros(k)
+        s, e = _r((0, 0), (k, k))
+        self._f("MultiVecInnerProdDiag", capi.F_MultiVecInnerProdDiag)(X.ptr, Y.ptr, _dp(out), s, e, self._p)
+        _check("MultiVecInnerProdDiag")
+        return out
+
+    def gram(self, V: MultiVec, W: MultiVec, vs=0, ve=None, ws=0, we=None, sym=False):
+        ve = V.ncols if ve is None else ve
+        we = W.ncols if we is None else we
+        p = (we - ws) if sym else (ve - vs)
+        q = we - ws
+        a = np.zeros((p, q), order="F")
+        s, e = _r((vs, ws), (ve, we))
+        self._f("MultiVecInnerProd", capi.F_MultiVecInnerProd)(V.ptr, W.ptr, _dp(a), b"sym" if sym else b"ns", s, e,
+                                                               p, self._p)
+        _check("MultiVecInnerProd")
+        return a
+
+    def lincomb(self, X: MultiVec, Y: MultiVec, coef, xs=0, ys=0, alpha=1.0, beta=0.0):
+        coef = np.asfortranarray(coef, dtype=np.float64)
+        p, q = coef.shape
+        s, e = _r((xs, ys), (xs + p, ys + q))
+        if alpha == 1.0 and beta == 0.0:
+            self._f("MultiVecLinearComb", capi.F_MultiVecLinearComb)(X.ptr, Y.ptr, s, e, _dp(coef), p, None, 0,
+                                                                     self._p)
+        else:
+            self._f("MultiVecLinearCombAxpby", capi.F_MultiVecLinearCombAxpby)(X.ptr, Y.ptr, s, e, _dp(coef), p,
+                                                                               alpha, beta, self._p)
+        _check("MultiVecLinearComb")
+
+    def swap(self, X: MultiVec, Y: MultiVec, xs, ys, k):
+        s, e = _r((xs, ys), (xs + k, ys + k))
+        self._f("MultiVecSwap", capi.F_MultiVecSwap)(X.ptr, Y.ptr, s, e, self._p)
+        _check("MultiVecSwap")
+
+    def copy(self, X: MultiVec, Y: MultiVec, xs=0, ys=0, k=None):
+        k = X.ncols - xs if k is None else k
+        s, e = _r((xs, ys), (xs + k, ys + k))
+        self._f("MultiVecCopy", capi.F_MultiVecCopy)(X.ptr, Y.ptr, s, e, self._p)
+        _check("MultiVecCopy")
+
+    def gather(self, X: MultiVec, idx, Y: MultiVec, ys=0):
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        self._f("MultiVecGather", capi.F_MultiVecGather)(X.ptr, _ip(idx), len(idx), Y.ptr, ys, self._p)
+        _check("MultiVecGather")
+
+    def cg_update(self, X: MultiVec, xs, P: MultiVec, W: MultiVec, R: MultiVec, k, alpha):
+        alpha = np.ascontiguousarray(alpha, dtype=np.float64)
+        rr = np.zeros(k)
+        self._f("BlockCGUpdate", capi.F_BlockCGUpdate)(X.ptr, xs, P.ptr, W.ptr, R.ptr, k, _dp(alpha), _dp(rr),
+                                                       self._p)
+        _check("BlockCGUpdate")
+        return rr
+
+    def residual_norms(self, A: Matrix, B, X: MultiVec, idx, lam):
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        k = len(idx)
+        rn, xn, bn = np.zeros(k), np.zeros(k), np.zeros(k)
+        self._f("MultiVecResidualNorms", capi.F_MultiVecResidualNorms)(
+            A.ptr, B.ptr if B is not None else None, X.ptr, _ip(idx), k, _dp(lam), _dp(rn), _dp(xn), _dp(bn),
+            self._p)
+        _check("MultiVecResidualNorms")
+        return rn, xn, bn
+
+    def dirichlet(self, X: MultiVec, nev, A: Matrix, B: Matrix):
+        self._f("SetDirichletBoundary", capi.F_SetDirichletBoundary)(X.ptr, nev, A.ptr, B.ptr)
+        _check("SetDirichletBoundary")
+
+    def close(self):
+        if self._p:
+            capi.load().GCGE_OPS_Free(C.byref(self._p))
+            self._p = None
+
+
+def synchronize():
+    capi.load().GCGE_HIP_Synchronize()
+    _check("synchronize")
+
+
+def solve(A, B, nev, X0=None, precision="fp64", mixed=None, seed=None, keep_evec=True, **params):
+    """Solve A x = lambda B x on the device.  Returns a dict with eval, evec,
+    res, num_iter, num_unlock, seconds (GCGE_Solve wall time, or the whole
+    mixed solve), bcg statistics and, for precision="mixed", the phase info."""
+    L = capi.load()
+    mA = A if isinstance(A, Matrix) else Matrix(A)
+    mB = None if B is None else (B if isinstance(B, Matrix) else Matrix(B))
+    args = [b"gcge"]
+    keep = []
+    for k, v in params.items():
+        if k in _CLI:
+            args += [("-gcge_" + k).encode(), str(v).encode()]
+    if X0 is not None:
+        args += [b"-gcge_given_init_evec", b"1"]
+    args += [b"-gcge_print_para", str(params.get("print_para", 0)).encode(),
+             b"-gcge_print_result", str(params.get("print_result", 0)).encode()]
+    argv = (C.c_char_p * len(args))(*args)
+    keep.append(argv)
+    if seed is not None:
+        import ctypes.util
+        libc = C.CDLL(ctypes.util.find_library("c"))
+        libc.srand(C.c_uint(seed))
+    solver = L.GCGE_HIP_Solver_Init(mA.ptr, mB.ptr if mB else None, nev, len(args), argv)
+    if not solver:
+        _check("GCGE_HIP_Solver_Init")
+        raise RuntimeError("GCGE_HIP_Solver_Init failed")
+    sp = solver.contents
+    para = sp.para.contents
+    for k in _DIRECT:
+        if k in params:
+            setattr(para, k, params[k])
+    n = mA.n
+    try:
+        if sp.workspace.contents.V is None:
+            raise RuntimeError("solver setup failed (missing operations)")
+        if X0 is not None:
+            X0 = np.asfortranarray(X0, dtype=np.float64)
+            L.HIP_MultiVecSetFromHost(sp.evec, 0, nev, _dp(X0), n)
+            _check("initial block")
+        info = {}
+        t0 = time.perf_counter()
+        if precision == "fp64":
+            L.GCGE_SOLVER_Solve(solver)
+        elif precision == "mixed":
+            o = capi.GCGE_HIP_MIXED_OPTS()
+            mixed = mixed or {}
+            o.fp32_tol = mixed.get("fp32_tol", 0.0)
+            o.fp32_max_it = mixed.get("fp32_max_it", 0)
+            o.fp64_max_it = mixed.get("fp64_max_it", 0)
+            mi = capi.GCGE_HIP_MIXED_INFO()
+            L.GCGE_HIP_SOLVER_SolveMixed(solver, C.byref(o), C.byref(mi))
+            info = dict(fp32_iters=mi.fp32_iters, fp64_iters=mi.fp64_iters, fp32_seconds=mi.fp32_seconds,
+                        fp64_seconds=mi.fp64_seconds, fp64_num_unlock=mi.fp64_num_unlock)
+        else:
+            raise ValueError(precision)
+        L.GCGE_HIP_Synchronize()
+        wall = time.perf_counter() - t0
+        _check("solve")
+        ev = np.array(sp.eval[:nev])
+        res = np.array(para.res[:nev])
+        X = None
+        if keep_evec:
+            X = np.zeros((n, nev), order="F")
+            L.HIP_MultiVecGetToHost(sp.evec, 0, nev, _dp(X), n)
+            _check("download evec")
+        out = dict(eval=ev, evec=X, res=res, num_iter=para.num_iter, num_unlock=para.num_unlock,
+                   seconds=wall if precision == "mixed" else para.solve_seconds, wall=wall,
+                   bcg_calls=para.bcg_calls, bcg_iters_total=para.bcg_iters_total, info=info)
+        pt = para.stat_para.contents.part_time_total.contents
+        out["part_time"] = {f: getattr(pt, f) for f, _ in capi.GCGE_PART_TIME._fields_}
+        return out
+    finally:
+        L.GCGE_SOLVER_Free_All(C.byref(solver))
+        if not isinstance(A, Matrix):
+            mA.close()
+        if mB is not None and not isinstance(B, Matrix):
+            mB.close()
+
+
+def eig_gradient(A: Matrix, X, weights, nparam, jrowptr, jnz, jval):
+    """d(sum_i w_i lambda_i)/d theta via GCGE_HIP_EigGradient (B = I)."""
+    L = capi.load()
+    k = X.shape[1]
+    mv = MultiVec(A, k, data=X)
+    w = np.ascontiguousarray(weights, dtype=np.float64)
+    jrowptr = np.ascontiguousarray(jrowptr, dtype=np.int32)
+    jnz = np.ascontiguousarray(jnz, dtype=np.int32)
+    jval = np.ascontiguousarray(jval, dtype=np.float64)
+    grad = np.zeros(nparam)
+    L.GCGE_HIP_EigGradient(A.ptr, mv.ptr, k, _dp(w), nparam, _ip(jrowptr), _ip(jnz), _dp(jval), _dp(grad))
+    _check("GCGE_HIP_EigGradient")
+    mv.close()
+    return grad

@@ -1,0 +1,138 @@
+/*
+ * gcge_app_hip.h -- the MI355X (gfx950) backend of GCGE, "app/hip".
+ *
+ * It plays the role app/csr plays in the reference (app/csr/gcge_app_csr.h):
+ * a matrix type, vector/multivector types and the functions that fill the
+ * GCGE_OPS table.  Data live in HBM: a HIP_MAT is a CSR matrix (int32 row
+ * pointers and column indices, FP64 values and, on demand, an FP32 copy) and
+ * a multivector is one column-major allocation with a padded leading
+ * dimension.  The void ** handle of a multivector is opaque; the
+ * solver reaches its columns only through the GCGE_OPS slots.
+ *
+ * Every function returns (or latches) errors instead of aborting: the
+ * GCGE_OPS slots return void, so the first HIP or argument error is kept and
+ * can be read with GCGE_HIP_GetLastError.  Once an error is latched, the
+ * backend skips further device launches until GCGE_HIP_ClearError.
+ */
+#ifndef GCGE_APP_HIP_H_
+#define GCGE_APP_HIP_H_
+
+#include "gcge_config.h"
+#include "gcge_ops.h"
+#include "gcge_solver.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct HIP_MAT_ HIP_MAT;
+
+enum { GCGE_HIP_FP64 = 0, GCGE_HIP_FP32 = 1 };
+
+/* ---- matrices (counterpart of CSR_MAT, gcge_app_csr.h:27-40) ---- */
+
+/* Upload a square CSR matrix: 0-based int32 rowptr[nrows+1], col[nnz],
+ * FP64 val[nnz].  Columns need not be sorted.  Returns NULL on error. */
+HIP_MAT *HIP_MatCreateFromCSR(GCGE_INT nrows, GCGE_INT ncols, GCGE_INT nnz, const GCGE_INT *rowptr,
+                              const GCGE_INT *col, const GCGE_DOUBLE *val);
+/* Read the text format of CSR_ReadMatFile (app/csr/gcge_app_csr.c:185-236):
+ * nrow, ncol, nnz, then rowptr, col, val, whitespace separated. */
+HIP_MAT *HIP_ReadMatFile(const char *filename);
+void HIP_MatDestroy(HIP_MAT **mat);
+GCGE_INT HIP_MatNumRows(const HIP_MAT *mat);
+GCGE_INT HIP_MatNumNonzeros(const HIP_MAT *mat);
+/* Replace the values (same sparsity pattern). */
+GCGE_INT HIP_MatSetValues(HIP_MAT *mat, const GCGE_DOUBLE *val);
+/* Storage precision of the multivectors and vectors created from this matrix
+ * from now on (GCGE_HIP_FP64 or GCGE_HIP_FP32).  Matrix products with FP32
+ * multivectors use the FP32 copy of the values. */
+GCGE_INT HIP_MatSetPrecision(HIP_MAT *mat, GCGE_INT precision);
+
+/* ---- multivectors ---- */
+
+GCGE_INT HIP_MultiVecCreate(void ***mv, GCGE_INT n_vec, HIP_MAT *mat);
+void HIP_MultiVecDestroy(void ***mv);
+/* host (column-major, leading dimension ldh, FP64) <-> columns [start, start+ncols) */
+GCGE_INT HIP_MultiVecSetFromHost(void **mv, GCGE_INT start, GCGE_INT ncols, const GCGE_DOUBLE *host,
+                                 GCGE_INT ldh);
+GCGE_INT HIP_MultiVecGetToHost(void **mv, GCGE_INT start, GCGE_INT ncols, GCGE_DOUBLE *host,
+                               GCGE_INT ldh);
+GCGE_INT HIP_MultiVecNumCols(void **mv);
+GCGE_INT HIP_MultiVecPrecision(void **mv);
+
+/* ---- registration (counterparts of gcge_app_csr.h:81-84) ---- */
+
+void GCGE_HIP_SetOps(GCGE_OPS *ops);
+void GCGE_SOLVER_SetHIPOps(GCGE_SOLVER *solver);
+/* Create a solver for A x = lambda B x (B may be NULL) with nev eigenpairs,
+ * parse -gcge_* options from argv, allocate eval/evec (FP64 unless
+ * HIP_MatSetPrecision chose FP32) and set the solver up, as
+ * GCGE_CSR_Solver_Init does (app/csr/gcge_app_csr.c:92-124). */
+GCGE_SOLVER *GCGE_HIP_Solver_Init(HIP_MAT *A, HIP_MAT *B, GCGE_INT num_eigenvalues, GCGE_INT argc,
+                                  char *argv[]);
+
+/* ---- mixed precision (no counterpart in the reference) ---- */
+
+typedef struct {
+    GCGE_DOUBLE fp32_tol;     /* tolerance of the FP32 phase, same criterion as para->conv_type */
+    GCGE_INT fp32_max_it;     /* outer-iteration budget of the FP32 phase                      */
+    GCGE_INT fp64_max_it;     /* outer-iteration budget of the FP64 refinement                 */
+} GCGE_HIP_MIXED_OPTS;
+
+typedef struct {
+    GCGE_INT fp32_iters;
+    GCGE_INT fp64_iters;
+    GCGE_DOUBLE fp32_seconds;
+    GCGE_DOUBLE fp64_seconds;
+    GCGE_INT fp64_num_unlock;
+} GCGE_HIP_MIXED_INFO;
+
+/* Solve with an FP32 phase (matrix values and multivectors in FP32, dense
+ * Rayleigh-Ritz and the residual check in FP64) followed by FP64 refinement
+ * sweeps of the same GCG iteration started from the FP32 eigenvectors
+ * (given_init_evec).  solver must have been created by GCGE_HIP_Solver_Init
+ * with FP64 evec; its para (nev, ev_tol, conv_type, ...) applies to the
+ * refinement.  When given_init_evec is set, evec holds the start block. */
+GCGE_INT GCGE_HIP_SOLVER_SolveMixed(GCGE_SOLVER *solver, const GCGE_HIP_MIXED_OPTS *opts,
+                                    GCGE_HIP_MIXED_INFO *info);
+
+/* ---- eigenvalue gradients (adjoint; no counterpart in the reference) ---- */
+
+/* For a matrix family A(theta) with a fixed pattern, let J be the sparse
+ * Jacobian of A's stored values with respect to theta, given by parameter:
+ * jrowptr[nparam+1], jnz[] (index of a stored nonzero of A), jval[]
+ * (d A_k / d theta_p).  For the pairs (lambda_i, X(:, i)), i < k, with
+ * weights w_i and B = I, writes grad[p] = sum_i w_i x_i^T (dA/dtheta_p) x_i,
+ * i.e. d(sum_i w_i lambda_i)/d theta for B-normalized eigenvectors.  The
+ * per-nonzero sampled outer product and the reduction onto theta run on the
+ * device; grad is a host array of nparam entries. */
+GCGE_INT GCGE_HIP_EigGradient(HIP_MAT *A, void **X, GCGE_INT k, const GCGE_DOUBLE *w,
+                              GCGE_INT nparam, const GCGE_INT *jrowptr, const GCGE_INT *jnz,
+                              const GCGE_DOUBLE *jval, GCGE_DOUBLE *grad);
+
+/* ---- errors, synchronisation, counters ---- */
+
+/* Returns the latched error code (0 = none) and copies its message. */
+GCGE_INT GCGE_HIP_GetLastError(char *msg, GCGE_INT len);
+void GCGE_HIP_ClearError(void);
+GCGE_INT GCGE_HIP_Synchronize(void);
+/* Device memory currently held by the backend, in bytes. */
+double GCGE_HIP_DeviceBytes(void);
+/* Kernel profiler: when on, every launch of a kernel class is bracketed by
+ * HIP events on the backend's stream and its algorithmic bytes/flops are
+ * recorded.  GCGE_HIP_GetKernelStats(kind, ...) returns the number of kernel
+ * classes (or -1 for a bad kind) and fills the class name, launch count,
+ * summed device milliseconds, algorithmic bytes and flops. */
+void GCGE_HIP_SetProfiling(GCGE_INT on);
+void GCGE_HIP_ResetKernelStats(void);
+GCGE_INT GCGE_HIP_GetKernelStats(GCGE_INT kind, char *name, GCGE_INT len, double *count, double *ms,
+                                 double *bytes, double *flops);
+/* Struct layout of this ABI for binding checks: writes up to n values
+ * (sizeof GCGE_PARA and selected member offsets, sizeof GCGE_ORTH_PARA,
+ * GCGE_WORKSPACE, GCGE_SOLVER, GCGE_OPS, ...) and returns how many. */
+GCGE_INT GCGE_HIP_AbiLayout(GCGE_INT *out, GCGE_INT n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

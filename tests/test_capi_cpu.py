@@ -1,0 +1,143 @@
+"""CPU checks of the C ABI (no GPU needed): the library loads, exports every
+function declared in include/*.h, its struct layouts match the bindings, and
+the host-side parts (parameters, option parsing, ops-table setup, the
+multiplicity rule) behave as the reference's."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from gcge_hip import capi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = capi.LIB_PATH
+
+
+def declared_functions():
+    names = set()
+    for h in os.listdir(os.path.join(ROOT, "include")):
+        if not h.endswith(".h"):
+            continue
+        txt = open(os.path.join(ROOT, "include", h)).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        # drop struct bodies (function-pointer members are not exports)
+        txt = re.sub(r"typedef struct[^{;]*\{.*?\}\s*\w+\s*;", "", txt, flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b((?:GCGE|HIP)_\w+)\s*\(", txt, flags=re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_function():
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB], capture_output=True, text=True, check=True).stdout
+    exported = {l.split()[-1] for l in out.splitlines() if " T " in l}
+    decl = declared_functions()
+    assert len(decl) > 55
+    missing = sorted(decl - exported)
+    assert not missing, missing
+    # and the ctypes binding knows all of them
+    assert not sorted(decl - set(capi.SIGNATURES)), sorted(decl - set(capi.SIGNATURES))
+
+
+def test_library_loads_and_binds():
+    L = capi.load()
+    for name in capi.exported_symbols():
+        assert getattr(L, name) is not None
+
+
+def test_abi_layout_matches_bindings():
+    lib_side, py_side = capi.abi_check()
+    assert lib_side == py_side
+
+
+def test_ops_table_has_reference_slot_order():
+    # the first 36 members are the reference's GCGE_OPS slots in order
+    # (src/gcge_ops.h:29-215); the block-kernel slots follow
+    names = [f for f, _ in capi.GCGE_OPS._fields_]
+    assert names[:8] == ["VecSetRandomValue", "MatDotVec", "VecAxpby", "VecInnerProd", "VecLocalInnerProd",
+                         "VecCreateByVec", "VecCreateByMat", "VecDestroy"]
+    assert names[35] == "ArrayScale"
+    assert capi.GCGE_OPS.ArrayScale.offset == 35 * 8
+
+
+def test_para_defaults_match_reference():
+    L = capi.load()
+    p = C.POINTER(capi.GCGE_PARA)()
+    L.GCGE_PARA_Create(C.byref(p))
+    P = p.contents
+    # src/gcge_para.c:25-99
+    assert (P.nev, P.ev_max_it, P.block_size, P.given_init_evec) == (1, 30, 0, 0)
+    assert P.ev_tol == 1e-8 and P.conv_type == b"R" and P.orth_type == b"B"
+    assert P.w_orth_type == b"multi" and P.p_orth_type == b"multi"
+    o = P.orth_para.contents
+    assert (o.orth_zero_tol, o.reorth_tol, o.scbgs_reorth_tol, o.criterion_tol) == (1e-16, 0.75, 1e-16, 1e-6)
+    assert (o.max_reorth_time, o.max_direct_orth_length) == (3, 1)
+    assert (P.cg_max_it, P.cg_rate, P.cg_type, P.if_use_bcg) == (20, 1e-2, 1, 1)
+    assert P.multi_tol == 0.2 and P.multi_tol_for_lock == 1e-6
+    args = [b"prog", b"-gcge_nev", b"12", b"-gcge_ev_tol", b"1e-10", b"-gcge_cg_rate", b"1e-3",
+            b"-gcge_w_orth_type", b"gs", b"-unknown", b"-gcge_block_size", b"4"]
+    argv = (C.c_char_p * len(args))(*args)
+    assert L.GCGE_PARA_SetFromCommandLine(p, len(args), argv) == 0
+    assert P.nev == 12 and P.ev_tol == 1e-10 and P.block_size == 4 and P.w_orth_type == b"gs"
+    assert P.cg_rate == 1e-3
+    L.GCGE_PARA_Setup(p)
+    assert P.num_unlock == 12
+    assert o.orth_zero_tol == 1e-16 + np.finfo(float).eps
+    assert [P.res[i] for i in range(12)] == [1.0] * 12
+    L.GCGE_PARA_Free(C.byref(p))
+
+
+def test_ops_setup_rejects_missing_block_kernels():
+    L = capi.load()
+    ops = C.POINTER(capi.GCGE_OPS)()
+    L.GCGE_OPS_Create(C.byref(ops))
+    assert L.GCGE_OPS_Setup(ops) == 1          # empty table: refused
+    L.GCGE_HIP_SetOps(ops)
+    assert L.GCGE_OPS_Setup(ops) == 0          # HIP backend fills every slot
+    for name, _ in capi.GCGE_OPS._fields_:
+        if name not in ("LinearSolver", "linear_solver_workspace", "Orthonormalization", "DenseMatCreate",
+                        "DenseMatDestroy"):
+            assert getattr(ops.contents, name), name
+    L.GCGE_OPS_Free(C.byref(ops))
+
+
+@pytest.mark.parametrize("ev,exp", [
+    ([1.0, 1.1, 1.15, 1.3, 2.0, 2.1], 4),    # nev=3: 1.15, 1.3 within 0.2 of eval[3]=1.3? ratio rule
+    ([1.0, 2.0, 3.0, 3.5, 3.55, 10.0], 5),
+])
+def test_check_eval_multiplicity_rule(ev, exp):
+    """dim_x = nev + #consecutive i >= nev with |eval[i]/eval[nev] - 1| < 0.2
+    (gcge_eigsol.c:696-718), checked against a direct evaluation."""
+    L = capi.load()
+    nev, end = 3, 6
+    ev = np.array(ev)
+    d = C.c_int(0)
+    L.GCGE_CheckEvalMultiplicity(nev, nev, end, C.byref(d), ev.ctypes.data_as(capi.DP))
+    t = nev
+    for i in range(nev, end):
+        if abs(abs(ev[t] / ev[nev]) - 1) < 0.2:
+            t += 1
+        else:
+            break
+    assert d.value == t
+
+
+def test_no_gpu_calls_fail_cleanly_without_device():
+    """Without a GPU (this container) device entry points latch an error and
+    return NULL instead of crashing."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    L = capi.load()
+    rp = np.array([0, 1], dtype=np.int32)
+    ci = np.array([0], dtype=np.int32)
+    v = np.array([2.0])
+    p = L.HIP_MatCreateFromCSR(1, 1, 1, rp.ctypes.data_as(capi.IP), ci.ctypes.data_as(capi.IP),
+                               v.ctypes.data_as(capi.DP))
+    assert not p
+    code, msg = capi.last_error()
+    assert code != 0 and msg
+    L.GCGE_HIP_ClearError()
+    assert capi.last_error()[0] == 0
