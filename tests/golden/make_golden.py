@@ -85,6 +85,10 @@ def cases():
     X = P.random_x0(L16.shape[0], 6, 13)
     X[:, 3] = X[:, 1]
     out.append(("lap2d16_dupx0", L16, None, 6, X, 1, 0, [], {}))
+    # a zero column: deflated for certain at the initial orthonormalization
+    X = P.random_x0(L16.shape[0], 6, 13)
+    X[:, 3] = 0.0
+    out.append(("lap2d16_zerox0", L16, None, 6, X, 1, 0, [], {}))
     return out
 
 
