@@ -39,6 +39,8 @@ extern "C" GCGE_INT GCGE_HIP_SOLVER_SolveMixed(GCGE_SOLVER *solver, const GCGE_H
     o.fp32_tol = (opts && opts->fp32_tol > 0) ? opts->fp32_tol : 1e-4;
     o.fp32_max_it = (opts && opts->fp32_max_it > 0) ? opts->fp32_max_it : P->ev_max_it;
     o.fp64_max_it = (opts && opts->fp64_max_it > 0) ? opts->fp64_max_it : P->ev_max_it;
+    o.stagnation_window = opts ? opts->stagnation_window : 0;
+    o.stagnation_ratio = (opts && opts->stagnation_ratio > 0) ? opts->stagnation_ratio : 0.7;
     GCGE_HIP_MIXED_INFO inf;
     std::memset(&inf, 0, sizeof(inf));
 
@@ -63,11 +65,9 @@ extern "C" GCGE_INT GCGE_HIP_SOLVER_SolveMixed(GCGE_SOLVER *solver, const GCGE_H
         op32->scbgs_reorth_tol -= DBL_EPSILON;
         s32->para->ev_tol = o.fp32_tol;
         s32->para->ev_max_it = o.fp32_max_it;
-        s32->para->stagnation_window = 3;
-        s32->para->stagnation_ratio = 0.7;
+        s32->para->stagnation_window = o.stagnation_window;
+        s32->para->stagnation_ratio = o.stagnation_ratio;
         s32->para->print_para = 0;
-        s32->para->print_result = 0;
-        s32->para->print_eval = 0;
     }
     s32->eval = (double *)calloc(nev, sizeof(double));
     void **evec32 = nullptr;

@@ -298,6 +298,8 @@ def solve(A, B, nev, X0=None, precision="fp64", mixed=None, seed=None, keep_evec
             o.fp32_tol = mixed.get("fp32_tol", 0.0)
             o.fp32_max_it = mixed.get("fp32_max_it", 0)
             o.fp64_max_it = mixed.get("fp64_max_it", 0)
+            o.stagnation_window = mixed.get("stagnation_window", 0)
+            o.stagnation_ratio = mixed.get("stagnation_ratio", 0.0)
             mi = capi.GCGE_HIP_MIXED_INFO()
             L.GCGE_HIP_SOLVER_SolveMixed(solver, C.byref(o), C.byref(mi))
             info = dict(fp32_iters=mi.fp32_iters, fp64_iters=mi.fp64_iters, fp32_seconds=mi.fp32_seconds,

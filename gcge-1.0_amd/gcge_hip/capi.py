@@ -107,7 +107,8 @@ class GCGE_SOLVER(C.Structure):
 
 
 class GCGE_HIP_MIXED_OPTS(C.Structure):
-    _fields_ = [("fp32_tol", c_double), ("fp32_max_it", c_int), ("fp64_max_it", c_int)]
+    _fields_ = [("fp32_tol", c_double), ("fp32_max_it", c_int), ("fp64_max_it", c_int),
+                ("stagnation_window", c_int), ("stagnation_ratio", c_double)]
 
 
 class GCGE_HIP_MIXED_INFO(C.Structure):

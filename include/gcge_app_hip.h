@@ -77,6 +77,9 @@ typedef struct {
     GCGE_DOUBLE fp32_tol;     /* tolerance of the FP32 phase, same criterion as para->conv_type */
     GCGE_INT fp32_max_it;     /* outer-iteration budget of the FP32 phase                      */
     GCGE_INT fp64_max_it;     /* outer-iteration budget of the FP64 refinement                 */
+    GCGE_INT stagnation_window; /* FP32 phase also stops when max_res has not dropped below
+                                   stagnation_ratio x its best for this many iterations (0: off) */
+    GCGE_DOUBLE stagnation_ratio;
 } GCGE_HIP_MIXED_OPTS;
 
 typedef struct {
