@@ -459,6 +459,23 @@ void compute_x(void **V, double *coef, void **X, int size_X, int size_V, GCGE_OP
     ops->MultiVecLinearComb(V, X, s, e, coef + (size_t)sx * size_V + sx, size_V, nullptr, 0, ops);
 }
 
+// FP32 work columns of the inner solves (bcg_precision == 1), created once
+// per workspace from A through the backend's precision-aware constructor.
+bool bcg32_ready(void *A, int cols, GCGE_OPS *ops, GCGE_WORKSPACE *ws)
+{
+    if (ws->bcg32[0] && ws->bcg32_cols >= cols) return true;
+    if (!ops->MultiVecCreateByMatPrecision) {
+        fprintf(stderr, "GCGE: bcg_precision 1 needs MultiVecCreateByMatPrecision\n");
+        return false;
+    }
+    for (int i = 0; i < 4; i++)
+        if (ws->bcg32[i]) ops->MultiVecDestroy(&ws->bcg32[i], ws->bcg32_cols, ops);
+    const int n = ws->CG_p_size > cols ? ws->CG_p_size : cols;
+    for (int i = 0; i < 4; i++) ops->MultiVecCreateByMatPrecision(&ws->bcg32[i], n, A, 1, ops);
+    ws->bcg32_cols = n;
+    return ws->bcg32[0] && ws->bcg32[1] && ws->bcg32[2] && ws->bcg32[3];
+}
+
 // GCGE_ComputeW, gcge_xpw.c:118-217: W_j starts from x_j (|lambda| <= 1) or
 // x_j / lambda; the right-hand side is lambda B x_j or B x_j.
 void compute_w(void *A, void *B, void **V, double *eval, GCGE_OPS *ops, GCGE_PARA *para, GCGE_WORKSPACE *ws)
@@ -502,6 +519,20 @@ void compute_w(void *A, void *B, void **V, double *eval, GCGE_OPS *ops, GCGE_PAR
             ops->RestoreVecForMultiVec(V, w_start + idx, &wc);
             ops->RestoreVecForMultiVec(ws->V_tmp, idx, &rhs);
         }
+    } else if (para->if_use_bcg == 1 && para->bcg_precision == 1) {
+        // inner solves on FP32 copies: R32 <- rhs, X32 <- W0, CG, W <- X32
+        if (!bcg32_ready(A, w_length, ops, ws)) return;
+        set2(ms, 0, 0);
+        set2(me, w_length, w_length);
+        ops->MultiVecCopy(ws->V_tmp, ws->bcg32[0], ms, me, ops);
+        set2(ms, w_start, 0);
+        set2(me, w_start + w_length, w_length);
+        ops->MultiVecCopy(V, ws->bcg32[1], ms, me, ops);
+        bcg(A, ws->bcg32[0], ws->bcg32[1], 0, w_length, ops, para, ws->bcg32[2], ws->bcg32[3], ws->subspace_dtmp,
+            ws->subspace_itmp);
+        set2(ms, 0, w_start);
+        set2(me, w_length, w_start + w_length);
+        ops->MultiVecCopy(ws->bcg32[1], V, ms, me, ops);
     } else if (para->if_use_bcg == 1) {
         bcg(A, ws->V_tmp, V, w_start, w_length, ops, para, ws->CG_p, ws->evec, ws->subspace_dtmp, ws->subspace_itmp);
     } else {
@@ -944,6 +975,8 @@ extern "C" void GCGE_WORKSPACE_Free(GCGE_WORKSPACE **workspace, GCGE_PARA *, GCG
         ops->MultiVecDestroy(&ws->V_tmp, ws->V_tmp_size, ops);
         ops->MultiVecDestroy(&ws->CG_p, ws->CG_p_size, ops);
     }
+    for (int i = 0; i < 4; i++)
+        if (ws->bcg32[i]) ops->MultiVecDestroy(&ws->bcg32[i], ws->bcg32_cols, ops);
     free(ws);
     *workspace = nullptr;
 }

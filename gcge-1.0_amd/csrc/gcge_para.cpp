@@ -141,6 +141,7 @@ extern "C" GCGE_INT GCGE_PARA_SetFromCommandLine(GCGE_PARA *p, GCGE_INT argc, ch
         else if (is("-gcge_opt_rr_eig_partly")) p->opt_rr_eig_partly = atoi(v);
         else if (is("-gcge_opt_bcast")) p->opt_bcast = atoi(v);
         else if (is("-gcge_opt_allgatherv")) p->opt_allgatherv = atoi(v);
+        else if (is("-gcge_bcg_precision")) p->bcg_precision = atoi(v);
         else if (std::strcmp(a, "-gcge_help") == 0) {
             usage = 1;
             break;

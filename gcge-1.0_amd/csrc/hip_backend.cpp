@@ -189,7 +189,40 @@ bool sync()
     if (!g_ctx.init) return true;
     bool ok = check(hipStreamSynchronize(g_ctx.stream), "hipStreamSynchronize");
     if (!g_pending.empty()) prof_resolve();
+    g_ctx.stage_off = 0;  // every staged copy has completed
     return ok;
+}
+
+void *stage_to_device(const void *host, size_t bytes)
+{
+    if (!ensure_ctx()) return nullptr;
+    const size_t need = (bytes + 255) & ~(size_t)255;
+    if (need > g_ctx.stage_cap) {
+        if (!sync()) return nullptr;
+        if (g_ctx.stage_h) (void)hipHostFree(g_ctx.stage_h);
+        if (g_ctx.stage_d) {
+            (void)hipFree(g_ctx.stage_d);
+            g_ctx.dev_bytes -= (double)g_ctx.stage_cap;
+        }
+        g_ctx.stage_h = nullptr;
+        g_ctx.stage_d = nullptr;
+        const size_t cap = std::max(need, (size_t)(8u << 20));
+        if (!check(hipHostMalloc((void **)&g_ctx.stage_h, cap, hipHostMallocDefault), "hipHostMalloc stage") ||
+            !check(hipMalloc((void **)&g_ctx.stage_d, cap), "hipMalloc stage")) {
+            g_ctx.stage_cap = 0;
+            return nullptr;
+        }
+        g_ctx.stage_cap = cap;
+        g_ctx.dev_bytes += (double)cap;
+        g_ctx.stage_off = 0;
+    }
+    if (g_ctx.stage_off + need > g_ctx.stage_cap && !sync()) return nullptr;
+    char *h = g_ctx.stage_h + g_ctx.stage_off;
+    char *d = g_ctx.stage_d + g_ctx.stage_off;
+    std::memcpy(h, host, bytes);
+    if (!check(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, g_ctx.stream), "stage H2D")) return nullptr;
+    g_ctx.stage_off += need;
+    return d;
 }
 
 size_t elem_size(int prec) { return prec == GCGE_HIP_FP32 ? sizeof(float) : sizeof(double); }
@@ -962,6 +995,18 @@ void s_MultiVecCopy(void **x, void **y, GCGE_INT *start, GCGE_INT *end, GCGE_OPS
                     std::min(1024, k - c0));
 }
 
+void s_MultiVecCreateByMatPrecision(void ***mv, GCGE_INT n_vec, void *mat, GCGE_INT precision, GCGE_OPS *)
+{
+    HIP_MAT *A = (HIP_MAT *)mat;
+    *mv = nullptr;
+    if (!A || A->magic != kMagicMat || (precision != GCGE_HIP_FP64 && precision != GCGE_HIP_FP32)) {
+        latch(143, "MultiVecCreateByMatPrecision: bad arguments");
+        return;
+    }
+    if (precision == GCGE_HIP_FP32 && !mat_val32(A)) return;
+    *mv = (void **)mv_new(A->n, n_vec, precision, A);
+}
+
 GCGE_INT s_MultiVecNumRows(void **x)
 {
     MV *m = as_mv(x);
@@ -1008,6 +1053,7 @@ extern "C" void GCGE_HIP_SetOps(GCGE_OPS *ops)
     ops->MultiVecGather = s_MultiVecGather;
     ops->MultiVecCopy = s_MultiVecCopy;
     ops->MultiVecNumRows = s_MultiVecNumRows;
+    ops->MultiVecCreateByMatPrecision = s_MultiVecCreateByMatPrecision;
     GCGE_SetDenseOps(ops);
 }
 

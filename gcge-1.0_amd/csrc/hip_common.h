@@ -74,12 +74,21 @@ struct Ctx {
     size_t hbuf_bytes = 0;
     double dev_bytes = 0.0;
     int num_cu = 0;
+    // host->device staging ring (pinned host side, device side)
+    char *stage_h = nullptr;
+    char *stage_d = nullptr;
+    size_t stage_cap = 0;
+    size_t stage_off = 0;
 };
 
 Ctx &ctx();
 bool ensure_ctx();
 void *scratch(size_t bytes);           // device scratch of at least bytes
 double *host_buf(size_t count);        // pinned host doubles
+// Copy small host data (coefficients) to device memory without blocking:
+// the bytes go through a pinned ring whose slots are reused only after the
+// next stream synchronisation.  Returns the device address.
+void *stage_to_device(const void *host, size_t bytes);
 bool sync();                           // stream synchronize + error check
 
 // ---- kernel profiler (opt-in): HIP events around each launch of a kernel

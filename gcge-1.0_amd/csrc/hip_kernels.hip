@@ -782,11 +782,8 @@ bool lincomb(int prec, int64_t n, const void *X, int64_t ldx, int p, void *Y, in
     std::vector<double> hc((size_t)p * q);
     for (int j = 0; j < q; j++)
         for (int i = 0; i < p; i++) hc[(size_t)j * p + i] = C[(int64_t)j * ldc + i];
-    double *dC = (double *)scratch(sizeof(double) * hc.size());
+    const double *dC = (const double *)stage_to_device(hc.data(), sizeof(double) * hc.size());
     if (!dC) return false;
-    if (!check(hipMemcpyAsync(dC, hc.data(), sizeof(double) * hc.size(), hipMemcpyHostToDevice, ctx().stream),
-               "H2D lincomb coef"))
-        return false;
     dim3 grid((unsigned)((n + kLR - 1) / kLR), (q + kLQ - 1) / kLQ);
     {
         ProfScope ps(K_LINCOMB, (double)n * (p + q + (beta != 0.0 ? q : 0)) * elem_size(prec),
@@ -798,9 +795,7 @@ bool lincomb(int prec, int64_t n, const void *X, int64_t ldx, int p, void *Y, in
             k_lincomb<float, float><<<grid, kBlock, 0, ctx().stream>>>(n, (const float *)X, ldx, p, (float *)Y,
                                                                        ldy, q, dC, alpha, beta);
     }
-    if (!check(hipGetLastError(), "k_lincomb")) return false;
-    // the host staging vector dies here: make sure the copy has completed
-    return sync();
+    return check(hipGetLastError(), "k_lincomb");
 }
 
 bool sampled_outer(int prec, HIP_MAT *A, const void *X, int64_t ldx, int k, const double *w, double *g_dev)

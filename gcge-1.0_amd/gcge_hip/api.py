@@ -22,7 +22,7 @@ _CLI = {"ev_max_it", "block_size", "given_init_evec", "ev_tol", "conv_type", "co
         "max_reorth_time", "max_direct_orth_length", "multi_tol", "multi_tol_for_lock", "if_use_bcg",
         "cg_max_it", "cg_rate", "cg_type", "print_eval", "print_part_time", "print_para", "print_result",
         "opt_rr_eig_partly", "x_orth_block_size", "w_orth_block_size", "if_use_cg", "print_cg_error",
-        "print_orth_zero", "scbgs_wself_max_reorth_time"}
+        "print_orth_zero", "scbgs_wself_max_reorth_time", "bcg_precision"}
 # members set directly after initialisation (read at solve time)
 _DIRECT = {"dirichlet_boundary", "stagnation_window", "stagnation_ratio"}
 
@@ -250,9 +250,14 @@ def synchronize():
 
 
 def solve(A, B, nev, X0=None, precision="fp64", mixed=None, seed=None, keep_evec=True, **params):
-    """Solve A x = lambda B x on the device.  Returns a dict with eval, evec,
-    res, num_iter, num_unlock, seconds (GCGE_Solve wall time, or the whole
-    mixed solve), bcg statistics and, for precision="mixed", the phase info."""
+    """Solve A x = lambda B x on the device.
+
+    precision: "fp64" (everything FP64), "mixed" (the block-CG inner solves in
+    FP32, the rest FP64) or "fp32+refine" (an all-FP32 phase, then FP64
+    refinement from its eigenvectors, GCGE_HIP_SOLVER_SolveMixed).
+    Returns a dict with eval, evec, res, num_iter, num_unlock, seconds
+    (GCGE_Solve wall time, or the whole two-phase solve), bcg statistics and,
+    for "fp32+refine", the phase info."""
     L = capi.load()
     mA = A if isinstance(A, Matrix) else Matrix(A)
     mB = None if B is None else (B if isinstance(B, Matrix) else Matrix(B))
@@ -293,6 +298,10 @@ def solve(A, B, nev, X0=None, precision="fp64", mixed=None, seed=None, keep_evec
         if precision == "fp64":
             L.GCGE_SOLVER_Solve(solver)
         elif precision == "mixed":
+            # FP32 inner block-CG solves, everything else FP64
+            para.bcg_precision = 1
+            L.GCGE_SOLVER_Solve(solver)
+        elif precision == "fp32+refine":
             o = capi.GCGE_HIP_MIXED_OPTS()
             mixed = mixed or {}
             o.fp32_tol = mixed.get("fp32_tol", 0.0)
@@ -317,7 +326,7 @@ def solve(A, B, nev, X0=None, precision="fp64", mixed=None, seed=None, keep_evec
             L.HIP_MultiVecGetToHost(sp.evec, 0, nev, _dp(X), n)
             _check("download evec")
         out = dict(eval=ev, evec=X, res=res, num_iter=para.num_iter, num_unlock=para.num_unlock,
-                   seconds=wall if precision == "mixed" else para.solve_seconds, wall=wall,
+                   seconds=wall if precision == "fp32+refine" else para.solve_seconds, wall=wall,
                    bcg_calls=para.bcg_calls, bcg_iters_total=para.bcg_iters_total, info=info)
         pt = para.stat_para.contents.part_time_total.contents
         out["part_time"] = {f: getattr(pt, f) for f, _ in capi.GCGE_PART_TIME._fields_}

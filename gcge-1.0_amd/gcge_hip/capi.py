@@ -64,6 +64,7 @@ class GCGE_PARA(C.Structure):
         ("bcg_iters_total", c_int), ("bcg_calls", c_int), ("spmv_columns", c_int),
         ("solve_seconds", c_double),
         ("stagnation_window", c_int), ("stagnation_ratio", c_double), ("stopped_on_stagnation", c_int),
+        ("bcg_precision", c_int),
     ]
 
 
@@ -77,7 +78,7 @@ class GCGE_WORKSPACE(C.Structure):
         ("last_dim_xpw", c_int), ("last_dim_x", c_int), ("conv_bs", c_int), ("unconv_bs", c_int),
         ("num_soft_locked_in_last_iter", c_int), ("dense_matrix", c_void_p),
         ("subspace_matrix", DP), ("subspace_evec", DP), ("subspace_dtmp", DP), ("subspace_itmp", IP),
-        ("eval", DP), ("orth_ind", IP),
+        ("eval", DP), ("orth_ind", IP), ("bcg32", c_void_p * 4), ("bcg32_cols", c_int),
     ]
 
 
@@ -93,6 +94,7 @@ OPS_SLOTS = [
     # block-kernel slots of this build
     "MultiVecAxpbyDiag", "MultiVecInnerProdDiag", "MultiVecLinearCombAxpby", "MatDotMultiVecInnerProdDiag",
     "BlockCGUpdate", "MultiVecResidualNorms", "MultiVecGather", "MultiVecCopy", "MultiVecNumRows",
+    "MultiVecCreateByMatPrecision",
 ]
 
 

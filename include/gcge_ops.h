@@ -131,6 +131,10 @@ typedef struct GCGE_OPS_ {
     void (*MultiVecCopy)(void **x, void **y, GCGE_INT *start, GCGE_INT *end, struct GCGE_OPS_ *ops);
     /* number of rows of a multivector (for host-side checks) */
     GCGE_INT (*MultiVecNumRows)(void **x);
+    /* create n_vec columns shaped by mat with the given storage precision
+     * (0: FP64, 1: FP32); used for the FP32 inner solves (bcg_precision) */
+    void (*MultiVecCreateByMatPrecision)(void ***multi_vec, GCGE_INT n_vec, void *mat, GCGE_INT precision,
+                                         struct GCGE_OPS_ *ops);
 } GCGE_OPS;
 
 void GCGE_OPS_Create(GCGE_OPS **ops);

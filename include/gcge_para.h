@@ -123,6 +123,13 @@ typedef struct GCGE_PARA_ {
     GCGE_INT stagnation_window;
     GCGE_DOUBLE stagnation_ratio;
     GCGE_INT stopped_on_stagnation; /* output */
+    /* input: storage precision of the block-CG inner solves that build W
+     * (0: same as the basis; 1: FP32 -- right-hand sides and initial guesses
+     * are rounded to FP32, the CG runs on FP32 vectors and FP32 matrix values
+     * with FP64-accumulated dots, and W is widened back before it is
+     * orthonormalized).  Needs a backend that can create FP32 multivectors
+     * (HIP_MatSetPrecision). */
+    GCGE_INT bcg_precision;
 } GCGE_PARA;
 
 void GCGE_PARA_Create(GCGE_PARA **para);

@@ -59,6 +59,10 @@ typedef struct GCGE_WORKSPACE_ {
     GCGE_DOUBLE *eval;
 
     GCGE_INT *orth_ind;
+
+    /* ---- members of this build (no counterpart in the reference) ---- */
+    void **bcg32[4];     /* FP32 block-CG work columns (rhs/residual, solution, p, w) */
+    GCGE_INT bcg32_cols; /* columns allocated in each of them                        */
 } GCGE_WORKSPACE;
 
 typedef struct GCGE_SOLVER_ {

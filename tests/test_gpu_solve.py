@@ -139,24 +139,30 @@ def test_random_init_uses_reference_sequence():
     assert abs(r["num_iter"] - o["info"]["num_iter"]) <= 1
 
 
-def test_mixed_precision_reaches_fp64_answer():
+@pytest.mark.parametrize("precision", ["mixed", "fp32+refine"])
+def test_mixed_precision_reaches_fp64_answer(precision):
+    """Both mixed forms end on the FP64 answer: eigenvalues within 1e-10 of
+    the FP64 solve and every pair at the FP64 tolerance (NS criterion, FP64
+    residual of the returned FP64 vectors)."""
     A, _ = P.vardiff3d(20, seed=5)
     nev = 12
     X0 = P.random_x0(A.shape[0], nev, 6)
     r64 = solve(A, None, nev, X0=X0, conv_type="NS", ev_tol=1e-8, ev_max_it=300)
-    rmx = solve(A, None, nev, X0=X0, conv_type="NS", ev_tol=1e-8, ev_max_it=300, precision="mixed",
+    rmx = solve(A, None, nev, X0=X0, conv_type="NS", ev_tol=1e-8, ev_max_it=300, precision=precision,
                 mixed=dict(fp32_tol=1e-4, fp32_max_it=300))
     assert rmx["num_unlock"] == 0
-    assert rmx["info"]["fp32_iters"] > 0
+    if precision == "fp32+refine":
+        assert rmx["info"]["fp32_iters"] > 0
     np.testing.assert_allclose(rmx["eval"], r64["eval"], rtol=EV_RTOL)
     assert np.all(ns_residual(A, None, rmx["evec"], rmx["eval"]) <= 1e-8 * 1.01)
 
 
-def test_mixed_generalized_problem():
+@pytest.mark.parametrize("precision", ["mixed", "fp32+refine"])
+def test_mixed_generalized_problem(precision):
     A, B = P.lap3d_mass(10)
     nev = 8
     X0 = P.random_x0(A.shape[0], nev, 7)
-    rmx = solve(A, B, nev, X0=X0, conv_type="NS", ev_tol=1e-9, ev_max_it=300, precision="mixed")
+    rmx = solve(A, B, nev, X0=X0, conv_type="NS", ev_tol=1e-9, ev_max_it=300, precision=precision)
     assert rmx["num_unlock"] == 0
     np.testing.assert_allclose(rmx["eval"], P.lap3d_mass_eigs(10, nev), rtol=EV_RTOL)
 
