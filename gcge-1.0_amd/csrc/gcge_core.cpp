@@ -388,8 +388,10 @@ void cg_single(void *A, void *b, void *x, GCGE_OPS *ops, GCGE_PARA *para, void *
 // (:289-293) starting from the identity, so after each sweep the active
 // columns are 0..num_unlock-1 -- the first num_unlock columns, not the
 // unconverged ones.  That rule is kept.
+// x_is_zero: the initial guess is exactly zero, so the first residual
+// r = b - A*0 is b itself and its matrix product is skipped.
 void bcg(void *A, void **RHS, void **V, int x_start, int x_length, GCGE_OPS *ops, GCGE_PARA *para, void **CG_P,
-         void **CG_W, double *dtmp, int *itmp)
+         void **CG_W, double *dtmp, int *itmp, bool x_is_zero = false)
 {
     if (x_length <= 0) return;
     double *rho1 = dtmp, *rho2 = rho1 + x_length, *error = rho2 + x_length;
@@ -400,12 +402,16 @@ void bcg(void *A, void **RHS, void **V, int x_start, int x_length, GCGE_OPS *ops
     GCGE_INT ms[2], me[2];
     std::vector<double> a(x_length), b(x_length), alpha(x_length), rr(x_length);
 
-    set2(ms, x_start, 0);
-    set2(me, x_start + x_length, x_length);
-    ops->MatDotMultiVec(A, V, CG_W, ms, me, ops);
+    if (!x_is_zero) {
+        set2(ms, x_start, 0);
+        set2(me, x_start + x_length, x_length);
+        ops->MatDotMultiVec(A, V, CG_W, ms, me, ops);
+        set2(ms, 0, 0);
+        set2(me, x_length, x_length);
+        ops->MultiVecAxpby(-1.0, CG_W, 1.0, RHS, ms, me, ops);
+    }
     set2(ms, 0, 0);
     set2(me, x_length, x_length);
-    ops->MultiVecAxpby(-1.0, CG_W, 1.0, RHS, ms, me, ops);
     ops->MultiVecInnerProdDiag(RHS, RHS, rho2, ms, me, ops);
     for (int idx = 0; idx < x_length; idx++) unlock[idx] = idx;
     int num_unlock = x_length;
@@ -520,19 +526,26 @@ void compute_w(void *A, void *B, void **V, double *eval, GCGE_OPS *ops, GCGE_PAR
             ops->RestoreVecForMultiVec(ws->V_tmp, idx, &rhs);
         }
     } else if (para->if_use_bcg == 1 && para->bcg_precision == 1) {
-        // inner solves on FP32 copies: R32 <- rhs, X32 <- W0, CG, W <- X32
+        // Defect correction: r0 = rhs - A W0 in FP64 (the first step of the
+        // reference's BCG), CG for the correction A E = r0 from E = 0 on FP32
+        // copies (the same CG iterates in exact arithmetic, and the same
+        // ||r_k|| / ||r_0|| stopping rule), then W = W0 + E in FP64.  FP32
+        // rounding is relative to the correction, which shrinks as X converges.
         if (!bcg32_ready(A, w_length, ops, ws)) return;
-        set2(ms, 0, 0);
-        set2(me, w_length, w_length);
-        ops->MultiVecCopy(ws->V_tmp, ws->bcg32[0], ms, me, ops);
         set2(ms, w_start, 0);
         set2(me, w_start + w_length, w_length);
-        ops->MultiVecCopy(V, ws->bcg32[1], ms, me, ops);
+        ops->MatDotMultiVec(A, V, ws->evec, ms, me, ops);
+        set2(ms, 0, 0);
+        set2(me, w_length, w_length);
+        ops->MultiVecAxpby(-1.0, ws->evec, 1.0, ws->V_tmp, ms, me, ops);
+        ops->MultiVecCopy(ws->V_tmp, ws->bcg32[0], ms, me, ops);
+        ops->MultiVecAxpby(0.0, ws->bcg32[1], 0.0, ws->bcg32[1], ms, me, ops);
         bcg(A, ws->bcg32[0], ws->bcg32[1], 0, w_length, ops, para, ws->bcg32[2], ws->bcg32[3], ws->subspace_dtmp,
-            ws->subspace_itmp);
+            ws->subspace_itmp, true);
+        ops->MultiVecCopy(ws->bcg32[1], ws->V_tmp, ms, me, ops);
         set2(ms, 0, w_start);
         set2(me, w_length, w_start + w_length);
-        ops->MultiVecCopy(ws->bcg32[1], V, ms, me, ops);
+        ops->MultiVecAxpby(1.0, ws->V_tmp, 1.0, V, ms, me, ops);
     } else if (para->if_use_bcg == 1) {
         bcg(A, ws->V_tmp, V, w_start, w_length, ops, para, ws->CG_p, ws->evec, ws->subspace_dtmp, ws->subspace_itmp);
     } else {

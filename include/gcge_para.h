@@ -124,11 +124,11 @@ typedef struct GCGE_PARA_ {
     GCGE_DOUBLE stagnation_ratio;
     GCGE_INT stopped_on_stagnation; /* output */
     /* input: storage precision of the block-CG inner solves that build W
-     * (0: same as the basis; 1: FP32 -- right-hand sides and initial guesses
-     * are rounded to FP32, the CG runs on FP32 vectors and FP32 matrix values
-     * with FP64-accumulated dots, and W is widened back before it is
-     * orthonormalized).  Needs a backend that can create FP32 multivectors
-     * (HIP_MatSetPrecision). */
+     * (0: same as the basis; 1: FP32 defect correction -- the initial
+     * residual r0 = b - A W0 is formed in FP64, the CG for the correction
+     * A E = r0 runs on FP32 vectors and FP32 matrix values with
+     * FP64-accumulated dots, and W = W0 + E is formed in FP64).  Needs the
+     * MultiVecCreateByMatPrecision slot. */
     GCGE_INT bcg_precision;
 } GCGE_PARA;
 
