@@ -25,7 +25,7 @@ OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "lib", "libgcge_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["hip_kernels.hip", "hip_backend.cpp", "gcge_core.cpp", "gcge_para.cpp", "gcge_dense.cpp",
+SOURCES = ["hip_kernels.hip", "hip_tiles.hip", "hip_backend.cpp", "gcge_core.cpp", "gcge_para.cpp", "gcge_dense.cpp",
            "gcge_mixed.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=gfx950", "-ffp-contract=off",

@@ -279,6 +279,97 @@ void stable_multi_orth(void **V, int start, int *end, void *B, GCGE_OPS *ops, GC
     multi_orth(V, start, end, B, ops, para, ws);
 }
 
+}  // namespace
+
+int gcge_chol_inverse(int n, double *S, int lds);  // gcge_dense.cpp
+
+namespace {
+
+// Block B-orthonormalization of W = V[start:end] against V[0:start] and
+// itself by projection and CholQR (w_orth_type "cholqr"; no counterpart in
+// the reference, whose "multi" recursion issues O(nw log nw) small products).
+// Two projection passes W -= V1 (V1^T B W), then two CholQR passes: the Gram
+// S = W^T B W is scaled to unit diagonal, factored S = R^T R in FP64 on the
+// host, and W <- W D R^{-1}.  A column whose B-norm is at or below
+// orth_zero_tol, on which the factorization breaks down, or whose R_jj is
+// below kCholDropTol (it is spanned by the columns before it to that
+// relative accuracy) is moved to the end and dropped, as the reference
+// deflates zero vectors (gcge_orthogonal.c:1991-2001).
+constexpr double kCholDropTol = 1e-4;  // smallest kept R_jj of the unit-diagonal Gram
+
+void cholqr_orth(void **V, int start, int *end, void *B, GCGE_OPS *ops, GCGE_PARA *para, GCGE_WORKSPACE *ws)
+{
+    GCGE_INT ms[2], me[2];
+    const double zero_tol = para->orth_para->orth_zero_tol;
+    if (start > 0) {
+        for (int pass = 0; pass < 2 && *end > start; pass++) {
+            set2(ms, 0, start);
+            set2(me, start, *end);
+            sub_orth(V, ms, me, B, ws->V_tmp, ws->subspace_dtmp, ops);
+        }
+    }
+    std::vector<double> S, C;
+    for (int pass = 0; pass < 2 && *end > start; pass++) {
+        bool factored = false;
+        while (!factored && *end > start) {
+            const int nw = *end - start;
+            void **T = V;
+            int tc = start;
+            if (B) {
+                set2(ms, start, 0);
+                set2(me, *end, nw);
+                ops->MatDotMultiVec(B, V, ws->V_tmp, ms, me, ops);
+                T = ws->V_tmp;
+                tc = 0;
+            }
+            S.assign((size_t)nw * nw, 0.0);
+            set2(ms, start, tc);
+            set2(me, *end, tc + nw);
+            ops->MultiVecInnerProd(V, T, S.data(), (char *)"sym", ms, me, nw, ops);
+            // drop columns with a vanishing B-norm first
+            int drop = -1;
+            for (int i = 0; i < nw && drop < 0; i++)
+                if (!(S[(size_t)i * nw + i] > zero_tol * zero_tol)) drop = i;
+            std::vector<double> d(nw);
+            if (drop < 0) {
+                for (int i = 0; i < nw; i++) d[i] = 1.0 / std::sqrt(S[(size_t)i * nw + i]);
+                for (int j = 0; j < nw; j++)
+                    for (int i = 0; i < nw; i++) S[(size_t)j * nw + i] *= d[i] * d[j];
+                const int info = gcge_chol_inverse(nw, S.data(), nw);
+                if (info < 0) {
+                    fprintf(stderr, "GCGE: CholQR needs dpotrf/dtrtri from the LAPACK library\n");
+                    abort();
+                }
+                if (info > 0) drop = info - 1;
+                // R_jj (the part of column j not spanned by the columns before
+                // it, relative to its norm) = 1 / (R^{-1})_jj
+                for (int j = 0; j < nw && drop < 0; j++)
+                    if (!(1.0 / S[(size_t)j * nw + j] > kCholDropTol)) drop = j;
+            }
+            if (drop >= 0) {
+                set2(ms, start + drop, *end - 1);
+                set2(me, start + drop + 1, *end);
+                ops->MultiVecSwap(V, V, ms, me, ops);
+                (*end)--;
+                if (para->orth_para->print_orth_zero == 1)
+                    printf("In CholQR orthonormalization, column %d is dependent; end = %d\n", start + drop, *end);
+                continue;
+            }
+            // C = D R^{-1} (upper triangular), W <- W C through V_tmp
+            C.assign((size_t)nw * nw, 0.0);
+            for (int j = 0; j < nw; j++)
+                for (int i = 0; i <= j; i++) C[(size_t)j * nw + i] = d[i] * S[(size_t)j * nw + i];
+            set2(ms, start, 0);
+            set2(me, *end, nw);
+            ops->MultiVecLinearComb(V, ws->V_tmp, ms, me, C.data(), nw, nullptr, 0, ops);
+            set2(ms, 0, start);
+            set2(me, nw, *end);
+            ops->MultiVecCopy(ws->V_tmp, V, ms, me, ops);
+            factored = true;
+        }
+    }
+}
+
 // GCGE_MultiOrthonormalizationInSubspace, gcge_orthogonal.c:2130-2204
 // (host FP64; the second product updates ldV rows as the reference's does).
 void multi_orth_insub(double *Vd, int ldV, int nrows, int start, int *end, GCGE_ORTH_PARA *op, GCGE_WORKSPACE *ws,
@@ -745,6 +836,11 @@ extern "C" void GCGE_CheckConvergence(void *A, void *B, GCGE_DOUBLE *eval, void 
             }
         }
     }
+    if (!para->lock_converged && unconv_bs > 0) {
+        // every pair stays in the W and P blocks until all have converged
+        for (int i = 0; i < para->nev; i++) unlock[i] = i;
+        unconv_bs = para->nev;
+    }
     para->num_unlock = unconv_bs;
     ws->unconv_bs = (unconv_bs < para->block_size) ? unconv_bs : para->block_size;
     para->max_res = max_res;
@@ -864,7 +960,10 @@ extern "C" void GCGE_Solve(void *A, void *B, GCGE_DOUBLE *eval, void **evec, GCG
     ws->dim_xpw = ws->dim_xp + ws->unconv_bs;
     {
         GCGE_TIMER(w_orth_time, para);
-        orth_gs(V, ws->dim_x, &ws->dim_xpw, Orth_mat, ops, para, ws->V_tmp, ws->subspace_dtmp);
+        if (std::strcmp(para->w_orth_type, "cholqr") == 0)
+            cholqr_orth(V, ws->dim_x, &ws->dim_xpw, Orth_mat, ops, para, ws);
+        else
+            orth_gs(V, ws->dim_x, &ws->dim_xpw, Orth_mat, ops, para, ws->V_tmp, ws->subspace_dtmp);
     }
     ws->dim_xp = nev;
     rr_matrix(RR_mat, V, ops, para, ws);
@@ -883,7 +982,11 @@ extern "C" void GCGE_Solve(void *A, void *B, GCGE_DOUBLE *eval, void **evec, GCG
     int best_iter = para->num_iter;
     para->stopped_on_stagnation = 0;
     while (para->num_unlock > 0 && para->num_iter < ev_max_it) {
-        if (para->stagnation_window > 0) {
+        if (para->stagnation_window > 0 && para->stagnation_start > 0.0 && best_res >= para->stagnation_start) {
+            // not watched yet: residuals of a random start first grow
+            best_res = para->max_res;
+            best_iter = para->num_iter;
+        } else if (para->stagnation_window > 0) {
             if (para->max_res < para->stagnation_ratio * best_res) {
                 best_res = para->max_res;
                 best_iter = para->num_iter;
@@ -909,6 +1012,8 @@ extern "C" void GCGE_Solve(void *A, void *B, GCGE_DOUBLE *eval, void **evec, GCG
             GCGE_TIMER(w_orth_time, para);
             if (std::strcmp(para->w_orth_type, "multi") == 0)
                 stable_multi_orth(V, ws->dim_xp, &ws->dim_xpw, Orth_mat, ops, para, ws);
+            else if (std::strcmp(para->w_orth_type, "cholqr") == 0)
+                cholqr_orth(V, ws->dim_xp, &ws->dim_xpw, Orth_mat, ops, para, ws);
             else
                 orth_gs(V, ws->dim_xp, &ws->dim_xpw, Orth_mat, ops, para, ws->V_tmp, ws->subspace_dtmp);
         }

@@ -27,6 +27,8 @@ typedef double (*nrm2_t)(int *, double *, int *);
 typedef void (*scal_t)(int *, double *, double *, int *);
 typedef void (*axpy_t)(int *, double *, double *, int *, double *, int *);
 typedef void (*copy_t)(int *, double *, int *, double *, int *);
+typedef void (*potrf_t)(const char *, int *, double *, int *, int *);
+typedef void (*trtri_t)(const char *, const char *, int *, double *, int *, int *);
 
 struct Lapack {
     bool ok = false;
@@ -38,6 +40,8 @@ struct Lapack {
     scal_t dscal_ = nullptr;
     axpy_t daxpy_ = nullptr;
     copy_t dcopy_ = nullptr;
+    potrf_t dpotrf_ = nullptr;  // used by the CholQR orthonormalization only
+    trtri_t dtrtri_ = nullptr;
 };
 
 Lapack &lapack()
@@ -61,6 +65,8 @@ Lapack &lapack()
     L.dscal_ = (scal_t)dlsym(h, "dscal_");
     L.daxpy_ = (axpy_t)dlsym(h, "daxpy_");
     L.dcopy_ = (copy_t)dlsym(h, "dcopy_");
+    L.dpotrf_ = (potrf_t)dlsym(h, "dpotrf_");
+    L.dtrtri_ = (trtri_t)dlsym(h, "dtrtri_");
     L.ok = L.dsyevx_ && L.dgemm_ && L.dsymm_ && L.ddot_ && L.dnrm2_ && L.dscal_ && L.daxpy_ && L.dcopy_;
     if (!L.ok) fprintf(stderr, "GCGE: LAPACK library %s lacks a required routine\n", name);
     return L;
@@ -159,4 +165,21 @@ extern "C" void GCGE_SetDenseOps(GCGE_OPS *ops)
     ops->ArrayAXPBY = d_axpby;
     ops->ArrayCopy = d_copy;
     ops->ArrayScale = d_scale;
+}
+
+// Upper Cholesky factor S = R^T R of the n x n symmetric positive definite
+// matrix S (column-major, leading dimension lds; upper triangle read), then
+// R^{-1} in place of the upper triangle.  Returns LAPACK's info: 0 on
+// success, k > 0 when the leading minor of order k is not positive
+// definite (column k-1 depends on the columns before it), -1 when the
+// library lacks the routines.
+int gcge_chol_inverse(int n, double *S, int lds)
+{
+    Lapack &L = lapack();
+    if (!L.ok || !L.dpotrf_ || !L.dtrtri_) return -1;
+    int info = 0;
+    L.dpotrf_("U", &n, S, &lds, &info);
+    if (info != 0) return info;
+    L.dtrtri_("U", "N", &n, S, &lds, &info);
+    return info;
 }

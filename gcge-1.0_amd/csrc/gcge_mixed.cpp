@@ -2,15 +2,18 @@
 // iteration followed by FP64 refinement sweeps started from its eigenvectors.
 //
 // FP32 phase: matrix values and every multivector in FP32, so the SpMM, the
-// block CG and the Gram / linear-combination tiles move half the bytes.  The
-// dense Rayleigh-Ritz problem, the in-subspace orthogonalization and the
+// block CG and the Gram / linear-combination tiles move half the bytes, and
+// W is B-orthonormalized by projection and CholQR (fp32_w_orth) instead of
+// the recursive block Gram-Schmidt.  The dense Rayleigh-Ritz problem, the
+// Cholesky factorizations, the in-subspace orthogonalization and the
 // residual check stay FP64 (the residual kernel reads FP32 vectors into FP64
 // arithmetic).  FP32 storage bounds the attainable residual, so the phase
 // stops at fp32_tol or when the largest residual stops decreasing
-// (stagnation_window), whichever comes first.
+// (stagnation_window, watched from stagnation_start on), whichever first.
 // FP64 phase: the FP32 eigenvectors are widened to FP64 and handed to the
 // same solver as the initial block (given_init_evec), which B-orthonormalizes
-// them and iterates to the caller's tolerance.
+// them and iterates to the caller's tolerance, optionally with FP32 inner
+// solves (refine_bcg_precision) and CholQR (refine_w_orth).
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -41,6 +44,10 @@ extern "C" GCGE_INT GCGE_HIP_SOLVER_SolveMixed(GCGE_SOLVER *solver, const GCGE_H
     o.fp64_max_it = (opts && opts->fp64_max_it > 0) ? opts->fp64_max_it : P->ev_max_it;
     o.stagnation_window = opts ? opts->stagnation_window : 0;
     o.stagnation_ratio = (opts && opts->stagnation_ratio > 0) ? opts->stagnation_ratio : 0.7;
+    o.fp32_w_orth = opts ? opts->fp32_w_orth : 1;
+    o.refine_w_orth = opts ? opts->refine_w_orth : 0;
+    o.refine_bcg_precision = opts ? opts->refine_bcg_precision : 0;
+    o.stagnation_start = opts ? opts->stagnation_start : 0.0;
     GCGE_HIP_MIXED_INFO inf;
     std::memset(&inf, 0, sizeof(inf));
 
@@ -67,7 +74,10 @@ extern "C" GCGE_INT GCGE_HIP_SOLVER_SolveMixed(GCGE_SOLVER *solver, const GCGE_H
         s32->para->ev_max_it = o.fp32_max_it;
         s32->para->stagnation_window = o.stagnation_window;
         s32->para->stagnation_ratio = o.stagnation_ratio;
+        s32->para->stagnation_start = o.stagnation_start;
         s32->para->print_para = 0;
+        s32->para->bcg_precision = 0;  // the phase is FP32 throughout
+        if (o.fp32_w_orth) s32->para->w_orth_type = (char *)"cholqr";
     }
     s32->eval = (double *)calloc(nev, sizeof(double));
     void **evec32 = nullptr;
@@ -96,9 +106,12 @@ extern "C" GCGE_INT GCGE_HIP_SOLVER_SolveMixed(GCGE_SOLVER *solver, const GCGE_H
     }
     GCGE_SOLVER_Free_All(&s32);
     const double t1 = now_s();
-    const int given = P->given_init_evec, maxit = P->ev_max_it;
+    const int given = P->given_init_evec, maxit = P->ev_max_it, bcgp = P->bcg_precision;
+    char *worth = P->w_orth_type;
     P->given_init_evec = 1;
     P->ev_max_it = o.fp64_max_it;
+    P->bcg_precision = o.refine_bcg_precision;
+    if (o.refine_w_orth) P->w_orth_type = (char *)"cholqr";
     P->num_iter = 0;
     P->num_unlock = nev;
     P->sum_res = 0.0;
@@ -115,6 +128,8 @@ extern "C" GCGE_INT GCGE_HIP_SOLVER_SolveMixed(GCGE_SOLVER *solver, const GCGE_H
     GCGE_HIP_Synchronize();
     P->given_init_evec = given;
     P->ev_max_it = maxit;
+    P->bcg_precision = bcgp;
+    P->w_orth_type = worth;
     inf.fp64_iters = P->num_iter;
     inf.fp64_seconds = now_s() - t1;
     inf.fp64_num_unlock = P->num_unlock;

@@ -84,6 +84,8 @@ extern "C" void GCGE_PARA_Create(GCGE_PARA **para)
     p->opt_rr_eig_partly = 0;
     p->opt_bcast = 1;
     p->opt_allgatherv = 0;
+    p->stagnation_start = 0.0;
+    p->lock_converged = 1;
     *para = p;
 }
 
@@ -142,6 +144,7 @@ extern "C" GCGE_INT GCGE_PARA_SetFromCommandLine(GCGE_PARA *p, GCGE_INT argc, ch
         else if (is("-gcge_opt_bcast")) p->opt_bcast = atoi(v);
         else if (is("-gcge_opt_allgatherv")) p->opt_allgatherv = atoi(v);
         else if (is("-gcge_bcg_precision")) p->bcg_precision = atoi(v);
+        else if (is("-gcge_lock_converged")) p->lock_converged = atoi(v);
         else if (std::strcmp(a, "-gcge_help") == 0) {
             usage = 1;
             break;
@@ -153,7 +156,7 @@ extern "C" GCGE_INT GCGE_PARA_SetFromCommandLine(GCGE_PARA *p, GCGE_INT argc, ch
     }
     if (usage) {
         printf("Usage: [-gcge_nev i] [-gcge_ev_max_it i] [-gcge_block_size i] [-gcge_given_init_evec i]\n"
-               "       [-gcge_ev_tol d] [-gcge_conv_type R|O|A|NS] [-gcge_w_orth_type multi|gs]\n"
+               "       [-gcge_ev_tol d] [-gcge_conv_type R|O|A|NS] [-gcge_w_orth_type multi|gs|cholqr]\n"
                "       [-gcge_orth_zero_tol d] [-gcge_reorth_tol d] [-gcge_scbgs_reorth_tol d]\n"
                "       [-gcge_max_reorth_time i] [-gcge_max_direct_orth_length i] [-gcge_cg_max_it i]\n"
                "       [-gcge_cg_rate d] [-gcge_cg_type 1|2] [-gcge_if_use_bcg 0|1] [-gcge_print_* i]\n");

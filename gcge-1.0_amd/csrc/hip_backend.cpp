@@ -163,9 +163,14 @@ const char *kclass_name(int k)
     return (k >= 0 && k < K_NUM) ? names[k] : "?";
 }
 
+// scopes currently open: their end events are not recorded yet, so a
+// synchronisation inside one must leave the pending list alone
+static int g_prof_open = 0;
+
 ProfScope::ProfScope(int k, double bytes, double flops) : kind(k), on(g_prof && g_ctx.init)
 {
     if (!on) return;
+    g_prof_open++;
     ProfEntry e;
     e.kind = k;
     e.a = ev_get();
@@ -178,7 +183,9 @@ ProfScope::ProfScope(int k, double bytes, double flops) : kind(k), on(g_prof && 
 
 ProfScope::~ProfScope()
 {
-    if (!on || g_pending.empty()) return;
+    if (!on) return;
+    g_prof_open--;
+    if (g_pending.empty()) return;
     ProfEntry &e = g_pending.back();
     if (e.b) (void)hipEventRecord(e.b, g_ctx.stream);
     if (g_pending.size() > 4096) sync();
@@ -188,7 +195,7 @@ bool sync()
 {
     if (!g_ctx.init) return true;
     bool ok = check(hipStreamSynchronize(g_ctx.stream), "hipStreamSynchronize");
-    if (!g_pending.empty()) prof_resolve();
+    if (!g_pending.empty() && g_prof_open == 0) prof_resolve();
     g_ctx.stage_off = 0;  // every staged copy has completed
     return ok;
 }
@@ -1152,7 +1159,11 @@ extern "C" GCGE_INT GCGE_HIP_AbiLayout(GCGE_INT *out, GCGE_INT n)
                           (GCGE_INT)sizeof(GCGE_SOLVER),
                           (GCGE_INT)sizeof(GCGE_OPS),
                           (GCGE_INT)offsetof(GCGE_OPS, ArrayScale),
-                          (GCGE_INT)offsetof(GCGE_OPS, MultiVecAxpbyDiag)};
+                          (GCGE_INT)offsetof(GCGE_OPS, MultiVecAxpbyDiag),
+                          (GCGE_INT)offsetof(GCGE_PARA, lock_converged),
+                          (GCGE_INT)sizeof(GCGE_HIP_MIXED_OPTS),
+                          (GCGE_INT)offsetof(GCGE_HIP_MIXED_OPTS, stagnation_start),
+                          (GCGE_INT)sizeof(GCGE_HIP_MIXED_INFO)};
     const GCGE_INT m = (GCGE_INT)(sizeof(v) / sizeof(v[0]));
     for (GCGE_INT i = 0; i < m && i < n; i++) out[i] = v[i];
     return m < n ? m : n;

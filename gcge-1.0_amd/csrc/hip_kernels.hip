@@ -312,136 +312,6 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(int64_t n, T *__restrict__
     if (threadIdx.x == 0) partial[(int64_t)j * gridDim.x + blockIdx.x] = bs;
 }
 
-// ---------------------------------------------------------------- Gram
-// partial[chunk][j][i] = sum over the chunk's rows of V(r, i0+i) W(r, j0+j)
-constexpr int kGT = 64;   // tile edge
-constexpr int kGR = 32;   // rows per LDS stage
-
-template <typename T, typename Acc>
-__global__ __launch_bounds__(kBlock) void k_gram(int64_t n, const T *__restrict__ V, int64_t ldv,
-                                                 int p, const T *__restrict__ W, int64_t ldw, int q,
-                                                 int64_t rows_per_chunk, double *__restrict__ partial)
-{
-    __shared__ T sV[kGR][kGT + 1];
-    __shared__ T sW[kGR][kGT + 1];
-    const int i0 = blockIdx.x * kGT, j0 = blockIdx.y * kGT;
-    const int64_t rb = (int64_t)blockIdx.z * rows_per_chunk;
-    const int64_t re = min(n, rb + rows_per_chunk);
-    const int tid = threadIdx.x, ti = tid & 15, tj = tid >> 4;
-    Acc acc[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; a++)
-#pragma unroll
-        for (int b = 0; b < 4; b++) acc[a][b] = Acc(0);
-    for (int64_t r0 = rb; r0 < re; r0 += kGR) {
-        for (int e = tid; e < kGR * kGT; e += kBlock) {
-            const int rr = e % kGR, cc = e / kGR;
-            const int64_t row = r0 + rr;
-            const int ci = i0 + cc, cj = j0 + cc;
-            sV[rr][cc] = (row < re && ci < p) ? V[(int64_t)ci * ldv + row] : T(0);
-            sW[rr][cc] = (row < re && cj < q) ? W[(int64_t)cj * ldw + row] : T(0);
-        }
-        __syncthreads();
-#pragma unroll 4
-        for (int rr = 0; rr < kGR; rr++) {
-            Acc av[4], bv[4];
-#pragma unroll
-            for (int a = 0; a < 4; a++) av[a] = (Acc)sV[rr][ti + 16 * a];
-#pragma unroll
-            for (int b = 0; b < 4; b++) bv[b] = (Acc)sW[rr][tj + 16 * b];
-#pragma unroll
-            for (int a = 0; a < 4; a++)
-#pragma unroll
-                for (int b = 0; b < 4; b++) acc[a][b] = fma(av[a], bv[b], acc[a][b]);
-        }
-        __syncthreads();
-    }
-    double *out = partial + (int64_t)blockIdx.z * p * q;
-#pragma unroll
-    for (int a = 0; a < 4; a++)
-#pragma unroll
-        for (int b = 0; b < 4; b++) {
-            const int i = i0 + ti + 16 * a, j = j0 + tj + 16 * b;
-            if (i < p && j < q) out[(int64_t)j * p + i] = (double)acc[a][b];
-        }
-}
-
-// G[e] = sum_c partial[c][e], e < pq; one thread per entry, fixed order
-__global__ __launch_bounds__(kBlock) void k_reduce_chunks(const double *__restrict__ partial,
-                                                          int nchunk, int64_t pq,
-                                                          double *__restrict__ G)
-{
-    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (e >= pq) return;
-    double acc = 0.0;
-    for (int c = 0; c < nchunk; c++) acc += partial[(int64_t)c * pq + e];
-    G[e] = acc;
-}
-
-// ---------------------------------------------------------------- LinComb
-// Y(:, j0+j) = beta Y + alpha sum_i X(:, i) C[j*p + i]; tile 128 rows x 64 cols
-constexpr int kLR = 128;
-constexpr int kLQ = 64;
-constexpr int kLK = 16;
-
-template <typename T, typename Acc>
-__global__ __launch_bounds__(kBlock) void k_lincomb(int64_t n, const T *__restrict__ X, int64_t ldx,
-                                                    int p, T *__restrict__ Y, int64_t ldy, int q,
-                                                    const double *__restrict__ C, double alpha,
-                                                    double beta)
-{
-    __shared__ T sX[kLK][kLR + 1];
-    __shared__ Acc sC[kLK][kLQ + 1];
-    const int64_t row0 = (int64_t)blockIdx.x * kLR;
-    const int j0 = blockIdx.y * kLQ;
-    const int tid = threadIdx.x, tr = tid & 15, tc = tid >> 4;
-    Acc acc[8][4];
-#pragma unroll
-    for (int a = 0; a < 8; a++)
-#pragma unroll
-        for (int b = 0; b < 4; b++) acc[a][b] = Acc(0);
-    for (int k0 = 0; k0 < p; k0 += kLK) {
-        for (int e = tid; e < kLK * kLR; e += kBlock) {
-            const int rr = e % kLR, kk = e / kLR;
-            const int64_t row = row0 + rr;
-            sX[kk][rr] = (row < n && k0 + kk < p) ? X[(int64_t)(k0 + kk) * ldx + row] : T(0);
-        }
-        for (int e = tid; e < kLK * kLQ; e += kBlock) {
-            const int kk = e % kLK, cc = e / kLK;
-            sC[kk][cc] = (k0 + kk < p && j0 + cc < q) ? (Acc)C[(int64_t)(j0 + cc) * p + k0 + kk] : Acc(0);
-        }
-        __syncthreads();
-#pragma unroll 4
-        for (int kk = 0; kk < kLK; kk++) {
-            Acc xv[8], cv[4];
-#pragma unroll
-            for (int a = 0; a < 8; a++) xv[a] = (Acc)sX[kk][tr + 16 * a];
-#pragma unroll
-            for (int b = 0; b < 4; b++) cv[b] = sC[kk][tc + 16 * b];
-#pragma unroll
-            for (int a = 0; a < 8; a++)
-#pragma unroll
-                for (int b = 0; b < 4; b++) acc[a][b] = fma(xv[a], cv[b], acc[a][b]);
-        }
-        __syncthreads();
-    }
-    const T al = (T)alpha, be = (T)beta;
-#pragma unroll
-    for (int b = 0; b < 4; b++) {
-        const int j = j0 + tc + 16 * b;
-        if (j >= q) continue;
-        T *yj = Y + (int64_t)j * ldy;
-#pragma unroll
-        for (int a = 0; a < 8; a++) {
-            const int64_t row = row0 + tr + 16 * a;
-            if (row < n) {
-                const T v = (T)acc[a][b];
-                yj[row] = (be == T(0)) ? ((al == T(1)) ? v : al * v) : be * yj[row] + al * v;
-            }
-        }
-    }
-}
-
 // ---------------------------------------------------------------- gradient
 // g_e = sum_i w_i x_i[row_e] x_i[col_e], one thread per stored entry
 template <typename T>
@@ -723,79 +593,6 @@ bool cg_update(int prec, int64_t n, void *x, int64_t ldx, const void *p, int64_t
         if (!reduce_to_host(partial, kc, nb, rr + c0)) return false;
     }
     return true;
-}
-
-bool gram(int prec, int64_t n, const void *V, int64_t ldv, int p, const void *W, int64_t ldw, int q,
-          double *G, int ldg, bool upper_only)
-{
-    (void)upper_only;
-    if (p <= 0 || q <= 0) return true;
-    if (failed()) return false;
-    const int pt = (p + kGT - 1) / kGT, qt = (q + kGT - 1) / kGT;
-    int nchunk = (int)std::min<int64_t>(kMaxChunks, std::max<int64_t>(1, (2048 + pt * qt - 1) / (pt * qt)));
-    int64_t rpc = (n + nchunk - 1) / nchunk;
-    rpc = ((rpc + kGR - 1) / kGR) * kGR;
-    if (rpc < kGR) rpc = kGR;
-    nchunk = (int)((n + rpc - 1) / rpc);
-    if (nchunk < 1) nchunk = 1;
-    const int64_t pq = (int64_t)p * q;
-    double *partial = (double *)scratch(sizeof(double) * ((size_t)nchunk * pq + pq));
-    if (!partial) return false;
-    double *dG = partial + (int64_t)nchunk * pq;
-    dim3 grid(pt, qt, nchunk);
-    {
-        ProfScope ps(K_GRAM, (double)n * (p + q) * elem_size(prec) + 8.0 * pq, 2.0 * (double)n * pq);
-        if (prec == GCGE_HIP_FP64)
-            k_gram<double, double><<<grid, kBlock, 0, ctx().stream>>>(n, (const double *)V, ldv, p,
-                                                                      (const double *)W, ldw, q, rpc, partial);
-        else
-            k_gram<float, float><<<grid, kBlock, 0, ctx().stream>>>(n, (const float *)V, ldv, p,
-                                                                    (const float *)W, ldw, q, rpc, partial);
-    }
-    if (!check(hipGetLastError(), "k_gram")) return false;
-    k_reduce_chunks<<<(int)((pq + kBlock - 1) / kBlock), kBlock, 0, ctx().stream>>>(partial, nchunk, pq, dG);
-    if (!check(hipGetLastError(), "k_reduce_chunks")) return false;
-    double *h = host_buf((size_t)pq);
-    if (!h) return false;
-    if (!check(hipMemcpyAsync(h, dG, sizeof(double) * pq, hipMemcpyDeviceToHost, ctx().stream), "D2H gram"))
-        return false;
-    if (!sync()) return false;
-    for (int j = 0; j < q; j++)
-        for (int i = 0; i < p; i++) G[(int64_t)j * ldg + i] = h[(int64_t)j * p + i];
-    return true;
-}
-
-bool lincomb(int prec, int64_t n, const void *X, int64_t ldx, int p, void *Y, int64_t ldy, int q,
-             const double *C, int ldc, double alpha, double beta)
-{
-    if (q <= 0 || n <= 0) return true;
-    if (failed()) return false;
-    if (p <= 0) {
-        // empty sum: Y = beta Y
-        std::vector<double> b(kMaxCols, beta), a(kMaxCols, 0.0);
-        for (int c0 = 0; c0 < q; c0 += kMaxCols)
-            launch_axpby(prec, n, Y, ldy, (char *)Y + (size_t)c0 * ldy * elem_size(prec), ldy,
-                         std::min(kMaxCols, q - c0), a.data(), b.data(), nullptr);
-        return !failed();
-    }
-    // pack C (p x q) contiguously and upload
-    std::vector<double> hc((size_t)p * q);
-    for (int j = 0; j < q; j++)
-        for (int i = 0; i < p; i++) hc[(size_t)j * p + i] = C[(int64_t)j * ldc + i];
-    const double *dC = (const double *)stage_to_device(hc.data(), sizeof(double) * hc.size());
-    if (!dC) return false;
-    dim3 grid((unsigned)((n + kLR - 1) / kLR), (q + kLQ - 1) / kLQ);
-    {
-        ProfScope ps(K_LINCOMB, (double)n * (p + q + (beta != 0.0 ? q : 0)) * elem_size(prec),
-                     2.0 * (double)n * p * q);
-        if (prec == GCGE_HIP_FP64)
-            k_lincomb<double, double><<<grid, kBlock, 0, ctx().stream>>>(n, (const double *)X, ldx, p,
-                                                                         (double *)Y, ldy, q, dC, alpha, beta);
-        else
-            k_lincomb<float, float><<<grid, kBlock, 0, ctx().stream>>>(n, (const float *)X, ldx, p, (float *)Y,
-                                                                       ldy, q, dC, alpha, beta);
-    }
-    return check(hipGetLastError(), "k_lincomb");
 }
 
 bool sampled_outer(int prec, HIP_MAT *A, const void *X, int64_t ldx, int k, const double *w, double *g_dev)

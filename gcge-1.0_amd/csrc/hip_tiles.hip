@@ -1,0 +1,648 @@
+// hip_tiles.hip -- Gram products G = V^T W and block linear combinations
+// Y = beta Y + alpha X C on the gfx950 matrix cores.
+//
+// Both work on column-major multivectors (leading dimension a multiple of
+// 256 elements, so every column is 2 KiB / 1 KiB aligned) and use the
+// 16x16x4 MFMA shapes: v_mfma_f64_16x16x4f64 for FP64 data and
+// v_mfma_f32_16x16x4f32 for FP32 data (FP32 accumulation).  Lane layout of
+// these instructions, lane l = 0..63 of a wavefront:
+//   A operand (16 x 4):  A[l % 16][l / 16]
+//   B operand (4 x 16):  B[l / 16][l % 16]
+//   D (16 x 16), slot t of lane l:  FP32  D[4 (l / 16) + t][l % 16]
+//                                   FP64  D[4 t + l / 16][l % 16]
+// (the FP64 result rows are interleaved; checked on the device).
+//
+// Gram: the n rows are cut into chunks; a workgroup owns one chunk and one
+// output tile, and the reduction index k of the MFMA runs over rows.  Each
+// lane loads R consecutive rows of one column as one aligned vector (32
+// bytes), so a superstep of the wavefront covers 4 R rows and MFMA step t
+// takes row  base + 4 R s + (l / 16) R + t  for both operands.  Partial
+// tiles go to a per-chunk buffer that a second kernel sums in a fixed order.
+// Very thin products (p <= 4, q <= 8 or the transpose) take a vector kernel
+// that keeps the whole p x q block in registers and reads V and W once.
+//
+// LinComb: a wavefront owns 32 rows of Y and walks the output columns in
+// blocks of 16; X supplies the A operand straight from HBM (16 consecutive
+// rows per lane group), C the B operand from a copy packed on the host in
+// operand order.  Thin combinations (p, q <= 8) take a vector kernel with C
+// in the kernel arguments and 16-byte row vectors.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "hip_common.h"
+
+namespace gh {
+
+namespace {
+
+constexpr int kThreads = 256;       // 4 wavefronts per workgroup
+constexpr int kGramTargetWG = 2048; // workgroups a Gram launch aims for
+
+typedef double dx4 __attribute__((ext_vector_type(4)));
+typedef float fx4 __attribute__((ext_vector_type(4)));
+typedef float fx8 __attribute__((ext_vector_type(8)));
+
+template <typename T>
+struct MT;
+
+template <>
+struct MT<double> {
+    using acc_t = dx4;
+    using vec_t = dx4;              // R rows of one column
+    static constexpr int R = 4;
+    __device__ static __forceinline__ acc_t mma(double a, double b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    // result row of slot t in lane group g = l / 16
+    __device__ static __forceinline__ int drow(int g, int t) { return 4 * t + g; }
+};
+
+template <>
+struct MT<float> {
+    using acc_t = fx4;
+    using vec_t = fx8;
+    static constexpr int R = 8;
+    __device__ static __forceinline__ acc_t mma(float a, float b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    __device__ static __forceinline__ int drow(int g, int t) { return 4 * g + t; }
+};
+
+// R consecutive rows [row, row + R) of one column, zero past `end`
+template <typename T>
+__device__ __forceinline__ void load_rows(const T *__restrict__ col, int64_t row, int64_t end,
+                                          T (&v)[MT<T>::R])
+{
+    constexpr int R = MT<T>::R;
+    if (row + R <= end) {
+        const typename MT<T>::vec_t x = *reinterpret_cast<const typename MT<T>::vec_t *>(col + row);
+#pragma unroll
+        for (int t = 0; t < R; t++) v[t] = x[t];
+    } else {
+#pragma unroll
+        for (int t = 0; t < R; t++) v[t] = (row + t < end) ? col[row + t] : T(0);
+    }
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// Workgroups are dealt round-robin to the 8 XCDs; this renumbering gives
+// each XCD a contiguous range of logical blocks, so the tiles of one row
+// chunk run together on one XCD and share its L2.
+__device__ __forceinline__ int xcd_logical_block()
+{
+    const int per = (int)(gridDim.x >> 3);
+    return (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+}
+
+// ---------------------------------------------------------------- Gram (MFMA)
+// Workgroup tile TP x TQ = (GP WP 16) x (GQ WQ 16); its 4 wavefronts are
+// arranged GP x GQ over the tile and GK = 4 / (GP GQ) ways over the rows.
+// partial[chunk][j * p + i] = sum over the chunk's rows of V(r, i) W(r, j).
+template <typename T, int WP, int WQ, int GP, int GQ>
+__global__ __launch_bounds__(kThreads) void k_gram_mfma(int64_t n, const T *__restrict__ V, int64_t ldv, int p,
+                                                        const T *__restrict__ W, int64_t ldw, int q, int tiles_p,
+                                                        int tiles_q, int nchunk, int64_t rows_per_chunk,
+                                                        int upper_only, double *__restrict__ partial)
+{
+    constexpr int GK = 4 / (GP * GQ);
+    constexpr int R = MT<T>::R;
+    constexpr int S = 4 * R;  // rows per wavefront superstep
+    constexpr int TP = GP * WP * 16, TQ = GQ * WQ * 16;
+    using acc_t = typename MT<T>::acc_t;
+    __shared__ double red[GK > 1 ? (GK - 1) * WP * WQ * 4 * 64 : 1];
+
+    const int ntile = tiles_p * tiles_q;
+    const int logical = xcd_logical_block();
+    if (logical >= ntile * nchunk) return;
+    const int chunk = logical / ntile, tile = logical % ntile;
+    const int i0t = (tile % tiles_p) * TP, j0t = (tile / tiles_p) * TQ;
+    if (upper_only && i0t > j0t + TQ - 1) return;  // tile entirely below the diagonal
+
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wp = wave % GP, wq = (wave / GP) % GQ, wk = wave / (GP * GQ);
+    const int i0 = i0t + wp * WP * 16, j0 = j0t + wq * WQ * 16;
+    const int m = lane & 15, kk = lane >> 4;
+    const int64_t rb = (int64_t)chunk * rows_per_chunk;
+    const int64_t re = min(n, rb + rows_per_chunk);
+
+    const T *vc[WP];
+    const T *wc[WQ];
+    bool vok[WP], wok[WQ];
+#pragma unroll
+    for (int x = 0; x < WP; x++) {
+        const int c = i0 + x * 16 + m;
+        vok[x] = c < p;
+        vc[x] = V + (int64_t)(vok[x] ? c : 0) * ldv;
+    }
+#pragma unroll
+    for (int y = 0; y < WQ; y++) {
+        const int c = j0 + y * 16 + m;
+        wok[y] = c < q;
+        wc[y] = W + (int64_t)(wok[y] ? c : 0) * ldw;
+    }
+
+    acc_t acc[WP][WQ];
+#pragma unroll
+    for (int x = 0; x < WP; x++)
+#pragma unroll
+        for (int y = 0; y < WQ; y++) acc[x][y] = acc_t{};
+
+    for (int64_t s = rb + (int64_t)wk * S; s < re; s += (int64_t)GK * S) {
+        const int64_t row = s + (int64_t)kk * R;
+        T a[WP][R], b[WQ][R];
+#pragma unroll
+        for (int x = 0; x < WP; x++) {
+            if (vok[x]) {
+                load_rows<T>(vc[x], row, re, a[x]);
+            } else {
+#pragma unroll
+                for (int t = 0; t < R; t++) a[x][t] = T(0);
+            }
+        }
+#pragma unroll
+        for (int y = 0; y < WQ; y++) {
+            if (wok[y]) {
+                load_rows<T>(wc[y], row, re, b[y]);
+            } else {
+#pragma unroll
+                for (int t = 0; t < R; t++) b[y][t] = T(0);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < R; t++)
+#pragma unroll
+            for (int x = 0; x < WP; x++)
+#pragma unroll
+                for (int y = 0; y < WQ; y++) acc[x][y] = MT<T>::mma(a[x][t], b[y][t], acc[x][y]);
+    }
+
+    // sum the GK row-split partial tiles in wavefront order:
+    // red[((wk - 1) * SLOTS + slot) * 64 + lane]
+    if constexpr (GK > 1) {
+        constexpr int SLOTS = WP * WQ * 4;
+        static_assert(GP * GQ == 1, "row-split Gram tiles use a single wavefront column");
+        if (wk > 0) {
+#pragma unroll
+            for (int x = 0; x < WP; x++)
+#pragma unroll
+                for (int y = 0; y < WQ; y++)
+#pragma unroll
+                    for (int t = 0; t < 4; t++)
+                        red[((size_t)(wk - 1) * SLOTS + (x * WQ + y) * 4 + t) * 64 + lane] = (double)acc[x][y][t];
+        }
+        __syncthreads();
+        if (wk > 0) return;
+    }
+
+    double *out = partial + (int64_t)chunk * p * q;
+#pragma unroll
+    for (int x = 0; x < WP; x++)
+#pragma unroll
+        for (int y = 0; y < WQ; y++)
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                double v = (double)acc[x][y][t];
+                if constexpr (GK > 1) {
+                    constexpr int SLOTS = WP * WQ * 4;
+#pragma unroll
+                    for (int w = 0; w < GK - 1; w++) v += red[((size_t)w * SLOTS + (x * WQ + y) * 4 + t) * 64 + lane];
+                }
+                const int i = i0 + x * 16 + MT<T>::drow(kk, t), j = j0 + y * 16 + m;
+                if (i < p && j < q) out[(int64_t)j * p + i] = v;
+            }
+}
+
+// ---------------------------------------------------------------- Gram (thin)
+// p <= PM, q <= QM; every thread keeps the PM x QM block of its rows in
+// registers.  partial[block][o] with o = j * ldo + i, or i * ldo + j when
+// transpose is set (the caller swapped V and W).
+template <typename T, int PM, int QM>
+__global__ __launch_bounds__(kThreads) void k_gram_thin(int64_t n, const T *__restrict__ V, int64_t ldv, int p,
+                                                        const T *__restrict__ W, int64_t ldw, int q, int transpose,
+                                                        int ldo, double *__restrict__ partial)
+{
+    __shared__ double sh[4][PM * QM];
+    double acc[PM][QM];
+#pragma unroll
+    for (int a = 0; a < PM; a++)
+#pragma unroll
+        for (int b = 0; b < QM; b++) acc[a][b] = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * kThreads;
+#pragma unroll 2
+    for (int64_t r = (int64_t)blockIdx.x * kThreads + threadIdx.x; r < n; r += stride) {
+        double v[PM], w[QM];
+#pragma unroll
+        for (int a = 0; a < PM; a++) v[a] = (a < p) ? (double)V[(int64_t)a * ldv + r] : 0.0;
+#pragma unroll
+        for (int b = 0; b < QM; b++) w[b] = (b < q) ? (double)W[(int64_t)b * ldw + r] : 0.0;
+#pragma unroll
+        for (int a = 0; a < PM; a++)
+#pragma unroll
+            for (int b = 0; b < QM; b++) acc[a][b] = fma(v[a], w[b], acc[a][b]);
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int a = 0; a < PM; a++)
+#pragma unroll
+        for (int b = 0; b < QM; b++) {
+            const double s = wave_sum(acc[a][b]);
+            if (lane == 0) sh[wave][a * QM + b] = s;
+        }
+    __syncthreads();
+    const int pq = p * q;
+    for (int e = threadIdx.x; e < PM * QM; e += kThreads) {
+        const int a = e / QM, b = e % QM;
+        if (a < p && b < q) {
+            const double s = (sh[0][e] + sh[1][e]) + (sh[2][e] + sh[3][e]);
+            const int o = transpose ? a * ldo + b : b * ldo + a;
+            partial[(int64_t)blockIdx.x * pq + o] = s;
+        }
+    }
+}
+
+// G[e] = sum_c partial[c][e]: 64 entries per workgroup, the 4 wavefronts take
+// every 4th chunk, combined in a fixed order.  With upper_only, entries
+// below the diagonal (i > j, e = j * p + i) are skipped.
+__global__ __launch_bounds__(kThreads) void k_sum_chunks(const double *__restrict__ partial, int nchunk, int p,
+                                                         int64_t pq, int upper_only, double *__restrict__ G)
+{
+    __shared__ double sh[4][64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t e = (int64_t)blockIdx.x * 64 + lane;
+    bool live = e < pq;
+    if (live && upper_only) live = (e % p) <= (e / p);
+    double acc = 0.0;
+    if (live)
+        for (int c = wave; c < nchunk; c += 4) acc += partial[(int64_t)c * pq + e];
+    sh[wave][lane] = acc;
+    __syncthreads();
+    if (wave == 0 && e < pq) G[e] = live ? (sh[0][lane] + sh[1][lane]) + (sh[2][lane] + sh[3][lane]) : 0.0;
+}
+
+// ---------------------------------------------------------------- LinComb (MFMA)
+// Cp[((kb * nqb) + jb) * 64 + l] = C(4 kb + l / 16, 16 jb + l % 16), zero
+// past p and q.  A wavefront owns rows [rb, rb + 32) as two 16-row blocks and
+// walks the output in groups of RN 16-column blocks; slot t of lane l holds
+// row drow(l / 16, t) of its block and column l % 16.
+template <typename T, int RN>
+__global__ __launch_bounds__(kThreads) void k_lincomb_mfma(int64_t n, const T *__restrict__ X, int64_t ldx, int p,
+                                                           T *__restrict__ Y, int64_t ldy, int q,
+                                                           const T *__restrict__ Cp, int nkb, int nqb, double alpha,
+                                                           double beta)
+{
+    using acc_t = typename MT<T>::acc_t;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int m = lane & 15, kk = lane >> 4;
+    const int64_t rb = ((int64_t)blockIdx.x * 4 + wave) * 32;
+    if (rb >= n) return;  // no workgroup barrier in this kernel
+    const int64_t ra0 = rb + m, ra1 = rb + 16 + m;
+    const bool ok0 = ra0 < n, ok1 = ra1 < n;
+    const T al = (T)alpha, be = (T)beta;
+    for (int jb0 = 0; jb0 < nqb; jb0 += RN) {
+        acc_t acc0[RN], acc1[RN];
+#pragma unroll
+        for (int r = 0; r < RN; r++) {
+            acc0[r] = acc_t{};
+            acc1[r] = acc_t{};
+        }
+#pragma unroll 2
+        for (int kb = 0; kb < nkb; kb++) {
+            const int k = kb * 4 + kk;
+            const bool kok = k < p;
+            const T *xk = X + (int64_t)(kok ? k : 0) * ldx;
+            const T a0 = (kok && ok0) ? xk[ra0] : T(0);
+            const T a1 = (kok && ok1) ? xk[ra1] : T(0);
+            const T *cb = Cp + ((int64_t)kb * nqb + jb0) * 64 + lane;
+#pragma unroll
+            for (int r = 0; r < RN; r++) {
+                if (jb0 + r < nqb) {
+                    const T b = cb[r * 64];
+                    acc0[r] = MT<T>::mma(a0, b, acc0[r]);
+                    acc1[r] = MT<T>::mma(a1, b, acc1[r]);
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RN; r++) {
+            const int j = (jb0 + r) * 16 + m;
+            if (jb0 + r >= nqb || j >= q) continue;
+            T *yj = Y + (int64_t)j * ldy;
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    const int64_t row = rb + 16 * h + MT<T>::drow(kk, t);
+                    if (row < n) {
+                        const T v = (T)(h == 0 ? acc0[r][t] : acc1[r][t]);
+                        yj[row] = (be == T(0)) ? ((al == T(1)) ? v : al * v) : be * yj[row] + al * v;
+                    }
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- LinComb (thin)
+// p <= PM, q <= QM: C travels in the kernel arguments; each thread owns VR
+// consecutive rows (one 16-byte vector per column) and forms
+// y_j = beta y_j + alpha sum_i x_i C(i, j), the sum as fma from zero in i order.
+struct ThinCoef {
+    double c[64];  // C(i, j) at j * p + i
+};
+
+template <typename T, int PM, int QM>
+__global__ __launch_bounds__(kThreads) void k_lincomb_thin(int64_t n, const T *__restrict__ X, int64_t ldx, int p,
+                                                           T *__restrict__ Y, int64_t ldy, int q, ThinCoef cf,
+                                                           double alpha, double beta)
+{
+    constexpr int VR = 16 / sizeof(T);
+    typedef T vec_t __attribute__((ext_vector_type(VR)));
+    T c[PM][QM];
+#pragma unroll
+    for (int i = 0; i < PM; i++)
+#pragma unroll
+        for (int j = 0; j < QM; j++) c[i][j] = (i < p && j < q) ? (T)cf.c[j * p + i] : T(0);
+    const T al = (T)alpha, be = (T)beta;
+    const int64_t stride = (int64_t)gridDim.x * kThreads * VR;
+    for (int64_t r = ((int64_t)blockIdx.x * kThreads + threadIdx.x) * VR; r < n; r += stride) {
+        T x[PM][VR];
+        const bool full = r + VR <= n;
+#pragma unroll
+        for (int i = 0; i < PM; i++) {
+            if (i < p) {
+                const T *xi = X + (int64_t)i * ldx + r;
+                if (full) {
+                    const vec_t v = *reinterpret_cast<const vec_t *>(xi);
+#pragma unroll
+                    for (int t = 0; t < VR; t++) x[i][t] = v[t];
+                } else {
+#pragma unroll
+                    for (int t = 0; t < VR; t++) x[i][t] = (r + t < n) ? xi[t] : T(0);
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < QM; j++) {
+            if (j >= q) break;
+            T *yj = Y + (int64_t)j * ldy + r;
+            T out[VR];
+#pragma unroll
+            for (int t = 0; t < VR; t++) {
+                T acc = T(0);
+#pragma unroll
+                for (int i = 0; i < PM; i++)
+                    if (i < p) acc = fma(x[i][t], c[i][j], acc);
+                out[t] = acc;
+            }
+            if (full) {
+                vec_t y;
+                if (be != T(0)) y = *reinterpret_cast<const vec_t *>(yj);
+#pragma unroll
+                for (int t = 0; t < VR; t++)
+                    y[t] = (be == T(0)) ? ((al == T(1)) ? out[t] : al * out[t]) : be * y[t] + al * out[t];
+                *reinterpret_cast<vec_t *>(yj) = y;
+            } else {
+#pragma unroll
+                for (int t = 0; t < VR; t++)
+                    if (r + t < n)
+                        yj[t] = (be == T(0)) ? ((al == T(1)) ? out[t] : al * out[t]) : be * yj[t] + al * out[t];
+            }
+        }
+    }
+}
+
+template <typename T>
+void launch_lincomb_thin(int64_t n, const void *X, int64_t ldx, int p, void *Y, int64_t ldy, int q,
+                         const ThinCoef &cf, double alpha, double beta)
+{
+    constexpr int VR = 16 / sizeof(T);
+    const int64_t threads = (n + VR - 1) / VR;
+    const int grid = (int)std::min<int64_t>(8192, std::max<int64_t>(1, (threads + kThreads - 1) / kThreads));
+    const int pm = p <= 1 ? 1 : p <= 2 ? 2 : p <= 4 ? 4 : 8;
+    const int qm = q <= 1 ? 1 : q <= 2 ? 2 : q <= 4 ? 4 : 8;
+#define GH_LT(P_, Q_)                                                                                          \
+    if (pm == P_ && qm == Q_) {                                                                                 \
+        k_lincomb_thin<T, P_, Q_><<<grid, kThreads, 0, ctx().stream>>>(n, (const T *)X, ldx, p, (T *)Y, ldy, q, \
+                                                                       cf, alpha, beta);                        \
+        return;                                                                                                 \
+    }
+    GH_LT(1, 1) GH_LT(1, 2) GH_LT(1, 4) GH_LT(1, 8)
+    GH_LT(2, 1) GH_LT(2, 2) GH_LT(2, 4) GH_LT(2, 8)
+    GH_LT(4, 1) GH_LT(4, 2) GH_LT(4, 4) GH_LT(4, 8)
+    GH_LT(8, 1) GH_LT(8, 2) GH_LT(8, 4) GH_LT(8, 8)
+#undef GH_LT
+}
+
+template <typename T, int PM, int QM>
+void launch_thin(int grid, int64_t n, const void *V, int64_t ldv, int p, const void *W, int64_t ldw, int q,
+                 int transpose, int ldo, double *partial)
+{
+    k_gram_thin<T, PM, QM><<<grid, kThreads, 0, ctx().stream>>>(n, (const T *)V, ldv, p, (const T *)W, ldw, q,
+                                                                transpose, ldo, partial);
+}
+
+template <typename T>
+void launch_thin_sized(int grid, int64_t n, const void *V, int64_t ldv, int p, const void *W, int64_t ldw, int q,
+                       int transpose, int ldo, double *partial)
+{
+    const int pm = p <= 1 ? 1 : p <= 2 ? 2 : 4;
+    const int qm = q <= 1 ? 1 : q <= 2 ? 2 : q <= 4 ? 4 : 8;
+#define GH_THIN(P_, Q_) \
+    if (pm == P_ && qm == Q_) return launch_thin<T, P_, Q_>(grid, n, V, ldv, p, W, ldw, q, transpose, ldo, partial);
+    GH_THIN(1, 1) GH_THIN(1, 2) GH_THIN(1, 4) GH_THIN(1, 8)
+    GH_THIN(2, 1) GH_THIN(2, 2) GH_THIN(2, 4) GH_THIN(2, 8)
+    GH_THIN(4, 1) GH_THIN(4, 2) GH_THIN(4, 4) GH_THIN(4, 8)
+#undef GH_THIN
+}
+
+struct GramShape {
+    int WP, WQ, GP, GQ;
+};
+
+GramShape gram_shape(int p, int q)
+{
+    if (p <= 16 && q <= 16) return {1, 1, 1, 1};
+    if (p <= 32 && q <= 32) return {2, 2, 1, 1};
+    return {2, 2, 2, 2};
+}
+
+template <typename T>
+void launch_gram_mfma(const GramShape &g, int grid, int64_t n, const void *V, int64_t ldv, int p, const void *W,
+                      int64_t ldw, int q, int tp, int tq, int nchunk, int64_t rpc, int upper, double *partial)
+{
+#define GH_GRAM(WP_, WQ_, GP_, GQ_)                                                                             \
+    if (g.WP == WP_ && g.WQ == WQ_ && g.GP == GP_ && g.GQ == GQ_) {                                             \
+        k_gram_mfma<T, WP_, WQ_, GP_, GQ_><<<grid, kThreads, 0, ctx().stream>>>(                                \
+            n, (const T *)V, ldv, p, (const T *)W, ldw, q, tp, tq, nchunk, rpc, upper, partial);                \
+        return;                                                                                                 \
+    }
+    GH_GRAM(1, 1, 1, 1)
+    GH_GRAM(2, 2, 1, 1)
+    GH_GRAM(2, 2, 2, 2)
+#undef GH_GRAM
+}
+
+bool aligned32(const void *p) { return ((uintptr_t)p & 31u) == 0; }
+
+}  // namespace
+
+bool gram(int prec, int64_t n, const void *V, int64_t ldv, int p, const void *W, int64_t ldw, int q, double *G,
+          int ldg, bool upper_only)
+{
+    if (p <= 0 || q <= 0) return true;
+    if (failed()) return false;
+    const size_t es = elem_size(prec);
+    const int64_t pq = (int64_t)p * q;
+    const bool thin = (p <= 4 && q <= 8) || (q <= 4 && p <= 8);
+    // launch geometry and scratch first: growing the scratch synchronizes the
+    // stream, which must not happen inside a profiled launch
+    int nchunk = 0;
+    int64_t rpc = 0;
+    int tp = 1, tq = 1;
+    GramShape g{1, 1, 1, 1};
+    if (thin) {
+        nchunk = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n + kThreads - 1) / kThreads));
+    } else {
+        if (!aligned32(V) || !aligned32(W) || (ldv * es) % 32 || (ldw * es) % 32) {
+            latch(160, "gram: operands are not 32-byte aligned");
+            return false;
+        }
+        g = gram_shape(p, q);
+        const int TP = g.GP * g.WP * 16, TQ = g.GQ * g.WQ * 16;
+        const int GK = 4 / (g.GP * g.GQ);
+        const int R = prec == GCGE_HIP_FP64 ? 4 : 8;
+        tp = (p + TP - 1) / TP;
+        tq = (q + TQ - 1) / TQ;
+        const int64_t unit = (int64_t)4 * R * GK;  // rows one workgroup step covers
+        const int64_t max_chunks = std::max<int64_t>(1, (n + unit - 1) / unit);
+        nchunk = (int)std::min<int64_t>({(int64_t)kMaxChunks, max_chunks,
+                                         std::max<int64_t>(1, (kGramTargetWG + tp * tq - 1) / (tp * tq))});
+        rpc = (n + nchunk - 1) / nchunk;
+        rpc = ((rpc + unit - 1) / unit) * unit;
+        nchunk = (int)((n + rpc - 1) / rpc);
+    }
+    double *partial = (double *)scratch(sizeof(double) * ((size_t)nchunk * pq + pq));
+    if (!partial) return false;
+    {
+        ProfScope ps(K_GRAM, (double)n * (p + q) * es + 8.0 * pq, 2.0 * (double)n * pq);
+        if (thin) {
+            // the kernel wants p <= 4: swap the operands and transpose the output otherwise
+            const bool swap = p > 4;
+            if (prec == GCGE_HIP_FP64) {
+                if (!swap) launch_thin_sized<double>(nchunk, n, V, ldv, p, W, ldw, q, 0, p, partial);
+                else launch_thin_sized<double>(nchunk, n, W, ldw, q, V, ldv, p, 1, p, partial);
+            } else {
+                if (!swap) launch_thin_sized<float>(nchunk, n, V, ldv, p, W, ldw, q, 0, p, partial);
+                else launch_thin_sized<float>(nchunk, n, W, ldw, q, V, ldv, p, 1, p, partial);
+            }
+        } else {
+            const int64_t blocks = (int64_t)tp * tq * nchunk;
+            const int grid = (int)(((blocks + 7) / 8) * 8);
+            if (prec == GCGE_HIP_FP64)
+                launch_gram_mfma<double>(g, grid, n, V, ldv, p, W, ldw, q, tp, tq, nchunk, rpc, upper_only ? 1 : 0,
+                                         partial);
+            else
+                launch_gram_mfma<float>(g, grid, n, V, ldv, p, W, ldw, q, tp, tq, nchunk, rpc, upper_only ? 1 : 0,
+                                        partial);
+        }
+    }
+    if (!check(hipGetLastError(), "k_gram")) return false;
+    double *dG = partial + (int64_t)nchunk * pq;
+    // the thin kernel writes every entry, so upper_only only prunes MFMA tiles
+    const int up = (!thin && upper_only) ? 1 : 0;
+    k_sum_chunks<<<(int)((pq + 63) / 64), kThreads, 0, ctx().stream>>>(partial, nchunk, p, pq, up, dG);
+    if (!check(hipGetLastError(), "k_sum_chunks")) return false;
+    double *h = host_buf((size_t)pq);
+    if (!h) return false;
+    if (!check(hipMemcpyAsync(h, dG, sizeof(double) * pq, hipMemcpyDeviceToHost, ctx().stream), "D2H gram"))
+        return false;
+    if (!sync()) return false;
+    for (int j = 0; j < q; j++) std::memcpy(G + (int64_t)j * ldg, h + (int64_t)j * p, sizeof(double) * p);
+    return true;
+}
+
+bool lincomb(int prec, int64_t n, const void *X, int64_t ldx, int p, void *Y, int64_t ldy, int q, const double *C,
+             int ldc, double alpha, double beta)
+{
+    if (q <= 0 || n <= 0) return true;
+    if (failed()) return false;
+    if (p <= 0) {
+        // empty sum: Y = beta Y
+        std::vector<double> b(kMaxCols, beta), a(kMaxCols, 0.0);
+        for (int c0 = 0; c0 < q; c0 += kMaxCols)
+            launch_axpby(prec, n, Y, ldy, (char *)Y + (size_t)c0 * ldy * elem_size(prec), ldy,
+                         std::min(kMaxCols, q - c0), a.data(), b.data(), nullptr);
+        return !failed();
+    }
+    if (p <= 8 && q <= 8) {
+        if (((uintptr_t)X & 15u) || ((uintptr_t)Y & 15u) || (ldx * elem_size(prec)) % 16 ||
+            (ldy * elem_size(prec)) % 16) {
+            latch(161, "lincomb: operands are not 16-byte aligned");
+            return false;
+        }
+        ThinCoef cf;
+        for (int j = 0; j < q; j++)
+            for (int i = 0; i < p; i++) cf.c[j * p + i] = C[(int64_t)j * ldc + i];
+        ProfScope ps(K_LINCOMB, (double)n * (p + q + (beta != 0.0 ? q : 0)) * elem_size(prec),
+                     2.0 * (double)n * p * q);
+        if (prec == GCGE_HIP_FP64) launch_lincomb_thin<double>(n, X, ldx, p, Y, ldy, q, cf, alpha, beta);
+        else launch_lincomb_thin<float>(n, X, ldx, p, Y, ldy, q, cf, alpha, beta);
+        return check(hipGetLastError(), "k_lincomb_thin");
+    }
+    const int nkb = (p + 3) / 4, nqb = (q + 15) / 16;
+    const size_t ncp = (size_t)nkb * nqb * 64;
+    const void *dC = nullptr;
+    if (prec == GCGE_HIP_FP64) {
+        std::vector<double> cp(ncp, 0.0);
+        for (int kb = 0; kb < nkb; kb++)
+            for (int jb = 0; jb < nqb; jb++)
+                for (int l = 0; l < 64; l++) {
+                    const int k = 4 * kb + l / 16, j = 16 * jb + l % 16;
+                    if (k < p && j < q) cp[((size_t)kb * nqb + jb) * 64 + l] = C[(int64_t)j * ldc + k];
+                }
+        dC = stage_to_device(cp.data(), sizeof(double) * ncp);
+    } else {
+        std::vector<float> cp(ncp, 0.0f);
+        for (int kb = 0; kb < nkb; kb++)
+            for (int jb = 0; jb < nqb; jb++)
+                for (int l = 0; l < 64; l++) {
+                    const int k = 4 * kb + l / 16, j = 16 * jb + l % 16;
+                    if (k < p && j < q) cp[((size_t)kb * nqb + jb) * 64 + l] = (float)C[(int64_t)j * ldc + k];
+                }
+        dC = stage_to_device(cp.data(), sizeof(float) * ncp);
+    }
+    if (!dC) return false;
+    const int grid = (int)((n + 127) / 128);
+    {
+        ProfScope ps(K_LINCOMB, (double)n * (p + q + (beta != 0.0 ? q : 0)) * elem_size(prec),
+                     2.0 * (double)n * p * q);
+#define GH_LC(T_, RN_)                                                                                         \
+    k_lincomb_mfma<T_, RN_><<<grid, kThreads, 0, ctx().stream>>>(n, (const T_ *)X, ldx, p, (T_ *)Y, ldy, q,    \
+                                                                 (const T_ *)dC, nkb, nqb, alpha, beta)
+        if (prec == GCGE_HIP_FP64) {
+            if (nqb <= 1) GH_LC(double, 1);
+            else if (nqb <= 2) GH_LC(double, 2);
+            else if (nqb <= 4) GH_LC(double, 4);
+            else GH_LC(double, 8);
+        } else {
+            if (nqb <= 1) GH_LC(float, 1);
+            else if (nqb <= 2) GH_LC(float, 2);
+            else if (nqb <= 4) GH_LC(float, 4);
+            else GH_LC(float, 8);
+        }
+#undef GH_LC
+    }
+    return check(hipGetLastError(), "k_lincomb");
+}
+
+}  // namespace gh

@@ -22,9 +22,9 @@ _CLI = {"ev_max_it", "block_size", "given_init_evec", "ev_tol", "conv_type", "co
         "max_reorth_time", "max_direct_orth_length", "multi_tol", "multi_tol_for_lock", "if_use_bcg",
         "cg_max_it", "cg_rate", "cg_type", "print_eval", "print_part_time", "print_para", "print_result",
         "opt_rr_eig_partly", "x_orth_block_size", "w_orth_block_size", "if_use_cg", "print_cg_error",
-        "print_orth_zero", "scbgs_wself_max_reorth_time", "bcg_precision"}
+        "print_orth_zero", "scbgs_wself_max_reorth_time", "bcg_precision", "lock_converged"}
 # members set directly after initialisation (read at solve time)
-_DIRECT = {"dirichlet_boundary", "stagnation_window", "stagnation_ratio"}
+_DIRECT = {"dirichlet_boundary", "stagnation_window", "stagnation_ratio", "stagnation_start"}
 
 
 def _dp(a):
@@ -249,15 +249,31 @@ def synchronize():
     _check("synchronize")
 
 
+# GCGE_HIP_MIXED_OPTS of the two-phase forms (0 / 0.0 select the library's defaults:
+# fp32_tol 1e-4, the caller's ev_max_it, stagnation ratio 0.7)
+MIXED_DEFAULTS = dict(fp32_tol=0.0, fp32_max_it=0, fp64_max_it=0, stagnation_window=0, stagnation_ratio=0.0,
+                      fp32_w_orth=1, refine_w_orth=0, refine_bcg_precision=1, stagnation_start=0.0)
+FP32_REFINE_DEFAULTS = dict(MIXED_DEFAULTS, fp32_w_orth=0, refine_bcg_precision=0)
+
+
 def solve(A, B, nev, X0=None, precision="fp64", mixed=None, seed=None, keep_evec=True, **params):
     """Solve A x = lambda B x on the device.
 
-    precision: "fp64" (everything FP64), "mixed" (the block-CG inner solves in
-    FP32, the rest FP64) or "fp32+refine" (an all-FP32 phase, then FP64
-    refinement from its eigenvectors, GCGE_HIP_SOLVER_SolveMixed).
-    Returns a dict with eval, evec, res, num_iter, num_unlock, seconds
-    (GCGE_Solve wall time, or the whole two-phase solve), bcg statistics and,
-    for "fp32+refine", the phase info."""
+    precision:
+      "fp64"         everything FP64 (the drop-in);
+      "fp64-cg32"    FP64, with the block-CG inner solves in FP32 as a
+                     correction of the FP64 residual (para.bcg_precision = 1);
+      "mixed"        GCGE_HIP_SOLVER_SolveMixed: an FP32 phase (FP32 matrix
+                     values and multivectors, CholQR W orthonormalization),
+                     then FP64 refinement sweeps from its eigenvectors with
+                     FP32 inner solves;
+      "fp32+refine"  the same two phases with the reference's own
+                     orthonormalization and FP64 inner solves in the
+                     refinement.
+    ``mixed`` overrides GCGE_HIP_MIXED_OPTS fields.  Returns a dict with
+    eval, evec, res, num_iter, num_unlock, seconds (GCGE_Solve wall time, or
+    the whole two-phase solve), bcg statistics and, for the two-phase forms,
+    the phase info."""
     L = capi.load()
     mA = A if isinstance(A, Matrix) else Matrix(A)
     mB = None if B is None else (B if isinstance(B, Matrix) else Matrix(B))
@@ -297,18 +313,15 @@ def solve(A, B, nev, X0=None, precision="fp64", mixed=None, seed=None, keep_evec
         t0 = time.perf_counter()
         if precision == "fp64":
             L.GCGE_SOLVER_Solve(solver)
-        elif precision == "mixed":
-            # FP32 inner block-CG solves, everything else FP64
+        elif precision == "fp64-cg32":
             para.bcg_precision = 1
             L.GCGE_SOLVER_Solve(solver)
-        elif precision == "fp32+refine":
+        elif precision in ("mixed", "fp32+refine"):
             o = capi.GCGE_HIP_MIXED_OPTS()
-            mixed = mixed or {}
-            o.fp32_tol = mixed.get("fp32_tol", 0.0)
-            o.fp32_max_it = mixed.get("fp32_max_it", 0)
-            o.fp64_max_it = mixed.get("fp64_max_it", 0)
-            o.stagnation_window = mixed.get("stagnation_window", 0)
-            o.stagnation_ratio = mixed.get("stagnation_ratio", 0.0)
+            opts = dict(MIXED_DEFAULTS if precision == "mixed" else FP32_REFINE_DEFAULTS)
+            opts.update(mixed or {})
+            for f, _ in capi.GCGE_HIP_MIXED_OPTS._fields_:
+                setattr(o, f, opts[f])
             mi = capi.GCGE_HIP_MIXED_INFO()
             L.GCGE_HIP_SOLVER_SolveMixed(solver, C.byref(o), C.byref(mi))
             info = dict(fp32_iters=mi.fp32_iters, fp64_iters=mi.fp64_iters, fp32_seconds=mi.fp32_seconds,
@@ -326,7 +339,7 @@ def solve(A, B, nev, X0=None, precision="fp64", mixed=None, seed=None, keep_evec
             L.HIP_MultiVecGetToHost(sp.evec, 0, nev, _dp(X), n)
             _check("download evec")
         out = dict(eval=ev, evec=X, res=res, num_iter=para.num_iter, num_unlock=para.num_unlock,
-                   seconds=wall if precision == "fp32+refine" else para.solve_seconds, wall=wall,
+                   seconds=wall if precision in ("mixed", "fp32+refine") else para.solve_seconds, wall=wall,
                    bcg_calls=para.bcg_calls, bcg_iters_total=para.bcg_iters_total, info=info)
         pt = para.stat_para.contents.part_time_total.contents
         out["part_time"] = {f: getattr(pt, f) for f, _ in capi.GCGE_PART_TIME._fields_}

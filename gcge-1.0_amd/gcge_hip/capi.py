@@ -64,7 +64,7 @@ class GCGE_PARA(C.Structure):
         ("bcg_iters_total", c_int), ("bcg_calls", c_int), ("spmv_columns", c_int),
         ("solve_seconds", c_double),
         ("stagnation_window", c_int), ("stagnation_ratio", c_double), ("stopped_on_stagnation", c_int),
-        ("bcg_precision", c_int),
+        ("bcg_precision", c_int), ("stagnation_start", c_double), ("lock_converged", c_int),
     ]
 
 
@@ -110,7 +110,8 @@ class GCGE_SOLVER(C.Structure):
 
 class GCGE_HIP_MIXED_OPTS(C.Structure):
     _fields_ = [("fp32_tol", c_double), ("fp32_max_it", c_int), ("fp64_max_it", c_int),
-                ("stagnation_window", c_int), ("stagnation_ratio", c_double)]
+                ("stagnation_window", c_int), ("stagnation_ratio", c_double), ("fp32_w_orth", c_int),
+                ("refine_w_orth", c_int), ("refine_bcg_precision", c_int), ("stagnation_start", c_double)]
 
 
 class GCGE_HIP_MIXED_INFO(C.Structure):
@@ -261,5 +262,7 @@ def abi_check():
             GCGE_PARA.res.offset, GCGE_PARA.stat_para.offset, GCGE_PARA.opt_allgatherv.offset,
             GCGE_PARA.stagnation_ratio.offset, C.sizeof(GCGE_ORTH_PARA), C.sizeof(GCGE_WORKSPACE),
             GCGE_WORKSPACE.unlock.offset, GCGE_WORKSPACE.eval.offset, C.sizeof(GCGE_SOLVER),
-            C.sizeof(GCGE_OPS), GCGE_OPS.ArrayScale.offset, GCGE_OPS.MultiVecAxpbyDiag.offset]
+            C.sizeof(GCGE_OPS), GCGE_OPS.ArrayScale.offset, GCGE_OPS.MultiVecAxpbyDiag.offset,
+            GCGE_PARA.lock_converged.offset, C.sizeof(GCGE_HIP_MIXED_OPTS),
+            GCGE_HIP_MIXED_OPTS.stagnation_start.offset, C.sizeof(GCGE_HIP_MIXED_INFO)]
     return list(out[:n]), mine

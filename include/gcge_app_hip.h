@@ -80,6 +80,13 @@ typedef struct {
     GCGE_INT stagnation_window; /* FP32 phase also stops when max_res has not dropped below
                                    stagnation_ratio x its best for this many iterations (0: off) */
     GCGE_DOUBLE stagnation_ratio;
+    GCGE_INT fp32_w_orth;     /* W orthonormalization of the FP32 phase: 1 CholQR ("cholqr"),
+                                 0 the caller's w_orth_type                                   */
+    GCGE_INT refine_w_orth;   /* the same choice for the FP64 refinement                      */
+    GCGE_INT refine_bcg_precision; /* 1: block CG of the refinement in FP32 as a correction
+                                      of the FP64 residual (bcg_precision), 0: FP64           */
+    GCGE_DOUBLE stagnation_start;  /* stagnation is watched once max_res is below this
+                                      (0: from the first iteration)                           */
 } GCGE_HIP_MIXED_OPTS;
 
 typedef struct {

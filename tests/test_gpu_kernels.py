@@ -162,6 +162,55 @@ def test_gram_against_oracle(ops, p, q):
     np.testing.assert_allclose(G, ref, rtol=RTOL_RED, atol=RTOL_RED * np.abs(ref).max())
 
 
+# thin kernel (p <= 4, q <= 8 and the transpose), each MFMA workgroup shape
+# (16x16, 32x32, 64x64 tiles), partial tiles, and a row count that is not a
+# multiple of any superstep, on 100,489 rows so the rows split into chunks
+BIG = P.lap2d(317)[0]
+GRAM_SHAPES = [(1, 1), (1, 8), (8, 1), (4, 4), (3, 7), (7, 3), (6, 6), (12, 13), (16, 16), (17, 5),
+               (25, 25), (32, 32), (33, 40), (64, 64), (100, 100), (200, 100)]
+
+
+@pytest.mark.parametrize("p,q", GRAM_SHAPES)
+def test_gram_tiles_large_n(ops, p, q):
+    n = BIG.shape[0]
+    V = rng(30 + p).standard_normal((n, p))
+    W = rng(60 + q).standard_normal((n, q))
+    mA = Matrix(BIG)
+    v = MultiVec(mA, p + 3, data=np.hstack([V, np.ones((n, 3))]))
+    w = MultiVec(mA, q + 2, data=np.hstack([np.ones((n, 2)), W]))
+    G = ops.gram(v, w, vs=0, ve=p, ws=2, we=2 + q)
+    ref = V.T @ W
+    np.testing.assert_allclose(G, ref, rtol=RTOL_RED, atol=RTOL_RED * np.sqrt(n) * 10)
+
+
+@pytest.mark.parametrize("p,q", [(2, 5), (12, 13), (40, 70)])
+def test_gram_fp32_tiles(ops, p, q):
+    """FP32 data, FP32 accumulation inside a chunk, FP64 across chunks."""
+    n = BIG.shape[0]
+    V = rng(90 + p).standard_normal((n, p)).astype(np.float32).astype(np.float64)
+    W = rng(91 + q).standard_normal((n, q)).astype(np.float32).astype(np.float64)
+    mA = Matrix(BIG)
+    v = MultiVec(mA, p, F32, data=V)
+    w = MultiVec(mA, q, F32, data=W)
+    G = ops.gram(v, w)
+    bound = 1e-4 * (np.abs(V).T @ np.abs(W))
+    assert np.all(np.abs(G - V.T @ W) <= bound)
+
+
+def test_gram_sym_large_tiles(ops):
+    """sym with 64x64 tiles: tiles below the diagonal are not computed."""
+    n, q = BIG.shape[0], 150
+    V = rng(92).standard_normal((n, q))
+    W = V @ rng(93).standard_normal((q, q)) * 1e-2 + V
+    mA = Matrix(BIG)
+    v = MultiVec(mA, q, data=V)
+    w = MultiVec(mA, q, data=W)
+    G = ops.gram(v, w, sym=True)
+    up = np.triu(V.T @ W)
+    ref = up + np.triu(up, 1).T
+    np.testing.assert_allclose(G, ref, rtol=RTOL_RED, atol=RTOL_RED * np.sqrt(n) * 10)
+
+
 def test_gram_sym_mirrors_upper_triangle(ops):
     A = MATS["lap3d"]
     n, q = A.shape[0], 20
@@ -189,6 +238,52 @@ def test_lincomb_against_oracle(ops, p, q):
     ops.lincomb(x, y, Cf)
     ref = O.multivec_lincomb(X, Cf)
     np.testing.assert_allclose(y.get(), ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("p,q,alpha,beta", [(1, 17, 1.0, 0.0), (3, 40, -1.0, 1.0), (16, 16, 0.5, 0.0),
+                                             (33, 130, -1.0, 1.0), (300, 100, 1.0, 0.0), (2, 3, 2.0, -0.5),
+                                             (8, 8, -1.0, 1.0), (5, 7, 1.0, 0.0), (1, 1, 3.0, 0.0),
+                                             (9, 8, -1.0, 1.0)])
+def test_lincomb_tiles_large_n(ops, p, q, alpha, beta):
+    n = BIG.shape[0]
+    X = rng(100 + p).standard_normal((n, p))
+    Y = rng(101 + q).standard_normal((n, q))
+    Cf = rng(102).standard_normal((p, q))
+    mA = Matrix(BIG)
+    x = MultiVec(mA, p + 1, data=np.hstack([np.ones((n, 1)), X]))
+    y = MultiVec(mA, q + 2, data=np.hstack([Y, np.ones((n, 2))]))
+    ops.lincomb(x, y, Cf, xs=1, ys=0, alpha=alpha, beta=beta)
+    ref = alpha * (X @ Cf) + (beta * Y if beta != 0.0 else 0.0)
+    got = y.get()
+    scale = np.abs(X) @ np.abs(Cf)
+    assert np.all(np.abs(got[:, :q] - ref) <= 1e-13 * (scale + np.abs(Y)) + 1e-300)
+    np.testing.assert_array_equal(got[:, q:], 1.0)
+
+
+@pytest.mark.parametrize("p,q,beta", [(3, 5, 0.0), (8, 2, 1.0)])
+def test_lincomb_fp32_thin(ops, p, q, beta):
+    n = BIG.shape[0]
+    X = rng(112).standard_normal((n, p)).astype(np.float32).astype(np.float64)
+    Y = rng(113).standard_normal((n, q)).astype(np.float32).astype(np.float64)
+    Cf = rng(114).standard_normal((p, q))
+    mA = Matrix(BIG)
+    x = MultiVec(mA, p, F32, data=X)
+    y = MultiVec(mA, q, F32, data=Y)
+    ops.lincomb(x, y, Cf, alpha=-1.0, beta=beta)
+    ref = beta * Y - X @ Cf.astype(np.float32).astype(np.float64)
+    assert np.all(np.abs(y.get() - ref) <= 1e-6 * (np.abs(X) @ np.abs(Cf) + np.abs(Y)))
+
+
+def test_lincomb_fp32_large_n(ops):
+    n, p, q = BIG.shape[0], 37, 70
+    X = rng(110).standard_normal((n, p)).astype(np.float32).astype(np.float64)
+    Cf = rng(111).standard_normal((p, q))
+    mA = Matrix(BIG)
+    x = MultiVec(mA, p, F32, data=X)
+    y = MultiVec(mA, q, F32)
+    ops.lincomb(x, y, Cf)
+    ref = X @ Cf.astype(np.float32).astype(np.float64)
+    assert np.all(np.abs(y.get() - ref) <= 1e-5 * (np.abs(X) @ np.abs(Cf)))
 
 
 def test_lincomb_axpby_projection(ops):

@@ -139,9 +139,12 @@ def test_random_init_uses_reference_sequence():
     assert abs(r["num_iter"] - o["info"]["num_iter"]) <= 1
 
 
-@pytest.mark.parametrize("precision", ["mixed", "fp32+refine"])
+MIXED_FORMS = ["fp64-cg32", "mixed", "fp32+refine"]
+
+
+@pytest.mark.parametrize("precision", MIXED_FORMS)
 def test_mixed_precision_reaches_fp64_answer(precision):
-    """Both mixed forms end on the FP64 answer: eigenvalues within 1e-10 of
+    """Every mixed form ends on the FP64 answer: eigenvalues within 1e-10 of
     the FP64 solve and every pair at the FP64 tolerance (NS criterion, FP64
     residual of the returned FP64 vectors)."""
     A, _ = P.vardiff3d(20, seed=5)
@@ -151,13 +154,13 @@ def test_mixed_precision_reaches_fp64_answer(precision):
     rmx = solve(A, None, nev, X0=X0, conv_type="NS", ev_tol=1e-8, ev_max_it=300, precision=precision,
                 mixed=dict(fp32_tol=1e-4, fp32_max_it=300))
     assert rmx["num_unlock"] == 0
-    if precision == "fp32+refine":
+    if precision != "fp64-cg32":
         assert rmx["info"]["fp32_iters"] > 0
     np.testing.assert_allclose(rmx["eval"], r64["eval"], rtol=EV_RTOL)
     assert np.all(ns_residual(A, None, rmx["evec"], rmx["eval"]) <= 1e-8 * 1.01)
 
 
-@pytest.mark.parametrize("precision", ["mixed", "fp32+refine"])
+@pytest.mark.parametrize("precision", MIXED_FORMS)
 def test_mixed_generalized_problem(precision):
     A, B = P.lap3d_mass(10)
     nev = 8
@@ -165,6 +168,23 @@ def test_mixed_generalized_problem(precision):
     rmx = solve(A, B, nev, X0=X0, conv_type="NS", ev_tol=1e-9, ev_max_it=300, precision=precision)
     assert rmx["num_unlock"] == 0
     np.testing.assert_allclose(rmx["eval"], P.lap3d_mass_eigs(10, nev), rtol=EV_RTOL)
+
+
+@pytest.mark.parametrize("withB", [False, True])
+def test_cholqr_w_orthonormalization(withB):
+    """w_orth_type "cholqr" (projection + CholQR, the orthonormalization of the
+    mixed path) reaches the same FP64 eigenpairs as the reference's "multi"."""
+    if withB:
+        A, B = P.lap3d_mass(10)
+        ref = P.lap3d_mass_eigs(10, 10)
+    else:
+        A, B = P.lap2d(32)
+        ref = P.lap2d_eigs(32, 10)
+    X0 = P.random_x0(A.shape[0], 10, 8)
+    r = solve(A, B, 10, X0=X0, conv_type="NS", ev_tol=1e-9, ev_max_it=300, w_orth_type="cholqr")
+    assert r["num_unlock"] == 0
+    np.testing.assert_allclose(r["eval"], ref, rtol=EV_RTOL)
+    assert np.all(ns_residual(A, B, r["evec"], r["eval"]) <= 1e-9 * 1.01)
 
 
 def test_dirichlet_example_configuration():
