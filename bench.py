@@ -7,11 +7,14 @@ field (sigma 1, correlation length 8 cells), B = I, lowest nev = 100
 eigenpairs, X0 ~ U[-1,1] (seeded), every pair converged to
 ||A x - lambda x|| / (|lambda| ||x||) <= 1e-8.
 
-A step is one complete mixed-precision solve from X0 (default mode: the
-block-CG inner solves that build W run in FP32 on FP32 matrix values, the
-basis, orthonormalization, Rayleigh-Ritz and residual checks stay FP64), with
-the matrix and X0 already in HBM.  After the timed steps one FP64
-solve from the same X0 is timed for the comparison.  The device kernels are
+A step is one complete mixed-precision solve from X0 with the matrix and X0
+already in HBM (default mode "mixed": an FP32 phase -- FP32 matrix values and
+multivectors, FP32 SpMM / block CG / MFMA Gram and LinComb tiles, CholQR W
+orthonormalization, FP64 dense Rayleigh-Ritz and residual checks -- then FP64
+refinement sweeps of GCG from the FP32 eigenvectors with FP32 inner solves).
+After the timed steps, FP64 solves from the same X0 are timed for the
+comparison: one with the same GCGE options as the mixed run, and one with the
+reference's defaults (the drop-in configuration).  The device kernels are
 timed with HIP events on the backend stream over the timed region; the CPU
 baseline is the reference built from its own sources (oracle/_ref/ref_solve)
 on one core, on a bounded sample (see ``cpu_baseline``).
@@ -144,11 +147,16 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--max-it", type=int, default=2000)
-    ap.add_argument("--fp32-tol", type=float, default=1e-4)
-    ap.add_argument("--mode", default="mixed", choices=["mixed", "fp32+refine"],
-                    help="mixed: FP32 inner block-CG solves, the rest FP64; fp32+refine: FP32 phase then "
-                         "FP64 refinement")
-    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--fp32-tol", type=float, default=1e-7)
+    ap.add_argument("--mode", default="mixed", choices=["mixed", "fp32+refine", "fp64-cg32"],
+                    help="mixed: FP32 phase (CholQR) then FP64 refinement with FP32 inner solves; "
+                         "fp32+refine: the same phases with the reference's orthonormalization and FP64 inner "
+                         "solves; fp64-cg32: FP64 with FP32 inner block-CG solves")
+    ap.add_argument("--stagnation-window", type=int, default=10)
+    ap.add_argument("--stagnation-start", type=float, default=1e-3)
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 solve (reference defaults)")
+    ap.add_argument("--no-fp64-same", action="store_true",
+                    help="skip the FP64 solve with the mixed path's orthonormalization (w_orth_type cholqr)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample-size", type=int, default=24)
     ap.add_argument("--cpu-sample-iters", type=int, default=3)
@@ -176,7 +184,9 @@ def main():
     log(f"upload {time.perf_counter() - t:.2f}s")
 
     common = dict(conv_type="NS", ev_tol=args.tol, ev_max_it=args.max_it)
-    mixed = dict(fp32_tol=args.fp32_tol, fp32_max_it=args.max_it, fp64_max_it=args.max_it)
+    mixed = dict(fp32_tol=args.fp32_tol, fp32_max_it=args.max_it, fp64_max_it=args.max_it,
+                 stagnation_window=args.stagnation_window, stagnation_ratio=0.7,
+                 stagnation_start=args.stagnation_start)
 
     def step(keep=False):
         return solve(mA, mB, nev, X0=X0, precision=args.mode, mixed=mixed, keep_evec=keep, **common)
@@ -220,7 +230,15 @@ def main():
         L.GCGE_HIP_SetProfiling(0)
         fp64 = dict(seconds=r64["wall"], iters=r64["num_iter"], unconverged=r64["num_unlock"],
                     eval=r64["eval"], stats=kernel_delta(a64, b64))
-        log(f"fp64: {r64['wall']:.2f}s, {r64['num_iter']} it, unconverged {r64['num_unlock']}")
+        log(f"fp64 (reference defaults): {r64['wall']:.2f}s, {r64['num_iter']} it, "
+            f"unconverged {r64['num_unlock']}")
+    fp64s = None
+    if not args.no_fp64_same and args.mode == "mixed":
+        r64s = solve(mA, mB, nev, X0=X0, precision="fp64", keep_evec=False, w_orth_type="cholqr", **common)
+        fp64s = dict(seconds=r64s["wall"], iters=r64s["num_iter"], unconverged=r64s["num_unlock"],
+                     eval=r64s["eval"])
+        log(f"fp64 with CholQR orthonormalization: {r64s['wall']:.2f}s, {r64s['num_iter']} it, "
+            f"unconverged {r64s['num_unlock']}")
 
     cpu = None
     if not args.no_cpu:
@@ -248,7 +266,8 @@ def main():
                                if B is None else f"{args.workload} size {args.size} (n={n}, nnz={nnz}+{B.nnz}), nev={nev}",
                    "n": n, "nnz": nnz, "nev": nev, "tol": args.tol,
                    "criterion": "||Ax-lambda Bx||/(|lambda| ||Bx||)", "x0": "U[-1,1] seeded",
-                   "fp32_tol": args.fp32_tol},
+                   "mode": args.mode, "fp32_tol": args.fp32_tol,
+                   "stagnation": [args.stagnation_window, 0.7, args.stagnation_start]},
         "converged": bool(converged),
         "max_ns_residual": float(np.max(ns)),
         "mixed": dict(mode=args.mode, iters=last["num_iter"], bcg_sweeps=last["bcg_iters_total"],
@@ -272,6 +291,11 @@ def main():
         line["mixed_vs_fp64"] = round(fp64["seconds"] / (ms_per_step * 1e-3), 3)
         line["max_rel_eig_diff_mixed_vs_fp64"] = float(np.max(np.abs(last["eval"] - fp64["eval"]) /
                                                            np.abs(fp64["eval"])))
+    if fp64s:
+        line["fp64_cholqr"] = dict(eigenpairs_per_s=round(nev / fp64s["seconds"], 4),
+                                   seconds=round(fp64s["seconds"], 3), iters=fp64s["iters"],
+                                   converged=fp64s["unconverged"] == 0)
+        line["mixed_vs_fp64_cholqr"] = round(fp64s["seconds"] / (ms_per_step * 1e-3), 3)
     print(json.dumps(line), flush=True)
 
 

@@ -17,6 +17,7 @@
 // reference.  Where the reference exchanges vector pointers between V and
 // evec/V_tmp (MultiVecSwap), this code copies: the columns that the swap
 // would hand back to evec/V_tmp are never read before being overwritten.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -281,92 +282,147 @@ void stable_multi_orth(void **V, int start, int *end, void *B, GCGE_OPS *ops, GC
 
 }  // namespace
 
-int gcge_chol_inverse(int n, double *S, int lds);  // gcge_dense.cpp
+int gcge_chol_inverse(int n, double *S, int lds);            // gcge_dense.cpp
+int gcge_sym_eig(int n, double *S, int lds, double *w, double *Z);
 
 namespace {
 
 // Block B-orthonormalization of W = V[start:end] against V[0:start] and
-// itself by projection and CholQR (w_orth_type "cholqr"; no counterpart in
-// the reference, whose "multi" recursion issues O(nw log nw) small products).
-// Two projection passes W -= V1 (V1^T B W), then two CholQR passes: the Gram
-// S = W^T B W is scaled to unit diagonal, factored S = R^T R in FP64 on the
-// host, and W <- W D R^{-1}.  A column whose B-norm is at or below
-// orth_zero_tol, on which the factorization breaks down, or whose R_jj is
-// below kCholDropTol (it is spanned by the columns before it to that
-// relative accuracy) is moved to the end and dropped, as the reference
-// deflates zero vectors (gcge_orthogonal.c:1991-2001).
-constexpr double kCholDropTol = 1e-4;  // smallest kept R_jj of the unit-diagonal Gram
+// itself (w_orth_type "cholqr"; no counterpart in the reference, whose
+// "multi" recursion issues O(nw log nw) small products).
+//
+// Each round: two projection passes W -= V1 (V1^T B W), then two
+// factorization passes on the Gram S = W^T B W scaled to unit diagonal
+// (D S D):
+//   - Cholesky D S D = R^T R in FP64 on the host, W <- W D R^{-1}, when every
+//     R_jj is at least sqrt(tau);
+//   - otherwise the eigendecomposition D S D = Q diag(theta) Q^T, keeping
+//     the directions with theta_i > tau theta_max: W <- W D Q_k theta_k^{-1/2}
+//     (the block shrinks, as the reference deflates dependent vectors,
+//     gcge_orthogonal.c:1991-2001).
+// tau = 1e4 basis_eps bounds the condition of the factored Gram, so the
+// second pass restores orthonormality to working accuracy.  A round ends
+// with a check of V1^T B W; another round runs while its largest entry
+// exceeds 1e3 basis_eps (at most three rounds).  Columns with a B-norm at or
+// below orth_zero_tol are dropped first.
+void orth_factor_pass(void **V, int start, int *end, void *B, GCGE_OPS *ops, GCGE_PARA *para,
+                      GCGE_WORKSPACE *ws)
+{
+    GCGE_INT ms[2], me[2];
+    const double zero_tol = para->orth_para->orth_zero_tol;
+    const double tau = 1e4 * para->basis_eps;
+    int nw = *end - start;
+    if (nw <= 0) return;
+    // drop vanishing columns (B-norms from the Gram diagonal)
+    std::vector<double> S, Z, w, C, d;
+    for (;;) {
+        nw = *end - start;
+        if (nw <= 0) return;
+        void **T = V;
+        int tc = start;
+        if (B) {
+            set2(ms, start, 0);
+            set2(me, *end, nw);
+            ops->MatDotMultiVec(B, V, ws->V_tmp, ms, me, ops);
+            T = ws->V_tmp;
+            tc = 0;
+        }
+        S.assign((size_t)nw * nw, 0.0);
+        set2(ms, start, tc);
+        set2(me, *end, tc + nw);
+        ops->MultiVecInnerProd(V, T, S.data(), (char *)"sym", ms, me, nw, ops);
+        int drop = -1;
+        for (int i = 0; i < nw && drop < 0; i++)
+            if (!(S[(size_t)i * nw + i] > zero_tol * zero_tol)) drop = i;
+        if (drop < 0) break;
+        set2(ms, start + drop, *end - 1);
+        set2(me, start + drop + 1, *end);
+        ops->MultiVecSwap(V, V, ms, me, ops);
+        (*end)--;
+        if (para->orth_para->print_orth_zero == 1)
+            printf("In CholQR orthonormalization, column %d vanishes; end = %d\n", start + drop, *end);
+    }
+    d.resize(nw);
+    for (int i = 0; i < nw; i++) d[i] = 1.0 / std::sqrt(S[(size_t)i * nw + i]);
+    for (int j = 0; j < nw; j++)
+        for (int i = 0; i < nw; i++) S[(size_t)j * nw + i] *= d[i] * d[j];
+    std::vector<double> S0(S);
+    int k = nw;
+    C.assign((size_t)nw * nw, 0.0);
+    const int info = gcge_chol_inverse(nw, S.data(), nw);
+    bool chol_ok = info == 0;
+    if (info < 0) {
+        fprintf(stderr, "GCGE: CholQR needs dpotrf/dtrtri from the LAPACK library\n");
+        abort();
+    }
+    if (chol_ok)
+        for (int j = 0; j < nw && chol_ok; j++) chol_ok = 1.0 / S[(size_t)j * nw + j] > std::sqrt(tau);
+    if (chol_ok) {
+        for (int j = 0; j < nw; j++)
+            for (int i = 0; i <= j; i++) C[(size_t)j * nw + i] = d[i] * S[(size_t)j * nw + i];
+    } else {
+        w.resize(nw);
+        Z.resize((size_t)nw * nw);
+        if (gcge_sym_eig(nw, S0.data(), nw, w.data(), Z.data()) != 0) {
+            fprintf(stderr, "GCGE: eigendecomposition of the W Gram failed\n");
+            abort();
+        }
+        const double tmax = w[nw - 1];
+        int first = 0;
+        while (first < nw && !(w[first] > tau * tmax)) first++;
+        k = nw - first;
+        // column c of C: D Z(:, first + c) / sqrt(theta)
+        for (int c = 0; c < k; c++) {
+            const double sc = 1.0 / std::sqrt(w[first + c]);
+            for (int i = 0; i < nw; i++) C[(size_t)c * nw + i] = d[i] * Z[(size_t)(first + c) * nw + i] * sc;
+        }
+        if (k < nw && para->orth_para->print_orth_zero == 1)
+            printf("In CholQR orthonormalization, %d dependent directions dropped; end = %d\n", nw - k, start + k);
+    }
+    if (k > 0) {
+        set2(ms, start, 0);
+        set2(me, *end, k);
+        ops->MultiVecLinearComb(V, ws->V_tmp, ms, me, C.data(), nw, nullptr, 0, ops);
+        set2(ms, 0, start);
+        set2(me, k, start + k);
+        ops->MultiVecCopy(ws->V_tmp, V, ms, me, ops);
+    }
+    *end = start + k;
+}
 
 void cholqr_orth(void **V, int start, int *end, void *B, GCGE_OPS *ops, GCGE_PARA *para, GCGE_WORKSPACE *ws)
 {
     GCGE_INT ms[2], me[2];
-    const double zero_tol = para->orth_para->orth_zero_tol;
-    if (start > 0) {
-        for (int pass = 0; pass < 2 && *end > start; pass++) {
-            set2(ms, 0, start);
-            set2(me, start, *end);
-            sub_orth(V, ms, me, B, ws->V_tmp, ws->subspace_dtmp, ops);
+    const double check_tol = 1e3 * para->basis_eps;
+    std::vector<double> G;
+    for (int round = 0; round < 3 && *end > start; round++) {
+        if (start > 0) {
+            for (int pass = 0; pass < 2 && *end > start; pass++) {
+                set2(ms, 0, start);
+                set2(me, start, *end);
+                sub_orth(V, ms, me, B, ws->V_tmp, ws->subspace_dtmp, ops);
+            }
         }
-    }
-    std::vector<double> S, C;
-    for (int pass = 0; pass < 2 && *end > start; pass++) {
-        bool factored = false;
-        while (!factored && *end > start) {
-            const int nw = *end - start;
-            void **T = V;
-            int tc = start;
-            if (B) {
-                set2(ms, start, 0);
-                set2(me, *end, nw);
-                ops->MatDotMultiVec(B, V, ws->V_tmp, ms, me, ops);
-                T = ws->V_tmp;
-                tc = 0;
-            }
-            S.assign((size_t)nw * nw, 0.0);
-            set2(ms, start, tc);
-            set2(me, *end, tc + nw);
-            ops->MultiVecInnerProd(V, T, S.data(), (char *)"sym", ms, me, nw, ops);
-            // drop columns with a vanishing B-norm first
-            int drop = -1;
-            for (int i = 0; i < nw && drop < 0; i++)
-                if (!(S[(size_t)i * nw + i] > zero_tol * zero_tol)) drop = i;
-            std::vector<double> d(nw);
-            if (drop < 0) {
-                for (int i = 0; i < nw; i++) d[i] = 1.0 / std::sqrt(S[(size_t)i * nw + i]);
-                for (int j = 0; j < nw; j++)
-                    for (int i = 0; i < nw; i++) S[(size_t)j * nw + i] *= d[i] * d[j];
-                const int info = gcge_chol_inverse(nw, S.data(), nw);
-                if (info < 0) {
-                    fprintf(stderr, "GCGE: CholQR needs dpotrf/dtrtri from the LAPACK library\n");
-                    abort();
-                }
-                if (info > 0) drop = info - 1;
-                // R_jj (the part of column j not spanned by the columns before
-                // it, relative to its norm) = 1 / (R^{-1})_jj
-                for (int j = 0; j < nw && drop < 0; j++)
-                    if (!(1.0 / S[(size_t)j * nw + j] > kCholDropTol)) drop = j;
-            }
-            if (drop >= 0) {
-                set2(ms, start + drop, *end - 1);
-                set2(me, start + drop + 1, *end);
-                ops->MultiVecSwap(V, V, ms, me, ops);
-                (*end)--;
-                if (para->orth_para->print_orth_zero == 1)
-                    printf("In CholQR orthonormalization, column %d is dependent; end = %d\n", start + drop, *end);
-                continue;
-            }
-            // C = D R^{-1} (upper triangular), W <- W C through V_tmp
-            C.assign((size_t)nw * nw, 0.0);
-            for (int j = 0; j < nw; j++)
-                for (int i = 0; i <= j; i++) C[(size_t)j * nw + i] = d[i] * S[(size_t)j * nw + i];
+        for (int pass = 0; pass < 2 && *end > start; pass++) orth_factor_pass(V, start, end, B, ops, para, ws);
+        if (start == 0 || *end <= start) return;
+        // check V1^T B W
+        const int nw = *end - start;
+        void **T = V;
+        int tc = start;
+        if (B) {
             set2(ms, start, 0);
             set2(me, *end, nw);
-            ops->MultiVecLinearComb(V, ws->V_tmp, ms, me, C.data(), nw, nullptr, 0, ops);
-            set2(ms, 0, start);
-            set2(me, nw, *end);
-            ops->MultiVecCopy(ws->V_tmp, V, ms, me, ops);
-            factored = true;
+            ops->MatDotMultiVec(B, V, ws->V_tmp, ms, me, ops);
+            T = ws->V_tmp;
+            tc = 0;
         }
+        G.assign((size_t)start * nw, 0.0);
+        set2(ms, 0, tc);
+        set2(me, start, tc + nw);
+        ops->MultiVecInnerProd(V, T, G.data(), (char *)"ns", ms, me, start, ops);
+        double gmax = 0.0;
+        for (double g : G) gmax = std::max(gmax, std::fabs(g));
+        if (gmax <= check_tol) return;
     }
 }
 
@@ -978,15 +1034,19 @@ extern "C" void GCGE_Solve(void *A, void *B, GCGE_DOUBLE *eval, void **evec, GCG
     }
     if (para->print_result || para->print_eval) GCGE_PrintIterationInfo(ws->eval, para);
 
-    double best_res = para->max_res;
+    double best_res = para->max_res, peak_res = para->max_res;
     int best_iter = para->num_iter;
+    bool watching = false;
     para->stopped_on_stagnation = 0;
     while (para->num_unlock > 0 && para->num_iter < ev_max_it) {
-        if (para->stagnation_window > 0 && para->stagnation_start > 0.0 && best_res >= para->stagnation_start) {
+        peak_res = std::max(peak_res, para->max_res);
+        if (para->stagnation_window > 0 && para->stagnation_start > 0.0 &&
+            !(para->max_res <= para->stagnation_start * peak_res) && !watching) {
             // not watched yet: residuals of a random start first grow
             best_res = para->max_res;
             best_iter = para->num_iter;
         } else if (para->stagnation_window > 0) {
+            watching = true;
             if (para->max_res < para->stagnation_ratio * best_res) {
                 best_res = para->max_res;
                 best_iter = para->num_iter;

@@ -183,3 +183,24 @@ int gcge_chol_inverse(int n, double *S, int lds)
     L.dtrtri_("U", "N", &n, S, &lds, &info);
     return info;
 }
+
+// All eigenpairs of the n x n symmetric matrix S (column-major, leading
+// dimension lds, upper triangle read; S is overwritten): eigenvalues in
+// ascending order in w[n], orthonormal eigenvectors in Z (n x n, ldz = n).
+// Returns LAPACK's info (0 on success), -1 without LAPACK.
+int gcge_sym_eig(int n, double *S, int lds, double *w, double *Z)
+{
+    Lapack &L = lapack();
+    if (!L.ok) return -1;
+    double vl = 0.0, vu = 0.0, abstol = 0.0;
+    int il = 1, iu = n, m = 0, info = 0, lwork = 8 * n, ldz = n;
+    double *work = (double *)malloc(sizeof(double) * (size_t)lwork);
+    int *iwork = (int *)malloc(sizeof(int) * 5 * (size_t)n);
+    int *ifail = (int *)malloc(sizeof(int) * (size_t)n);
+    L.dsyevx_("V", "A", "U", &n, S, &lds, &vl, &vu, &il, &iu, &abstol, &m, w, Z, &ldz, work, &lwork, iwork, ifail,
+              &info);
+    free(work);
+    free(iwork);
+    free(ifail);
+    return info;
+}

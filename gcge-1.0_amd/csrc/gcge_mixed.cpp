@@ -14,6 +14,7 @@
 // same solver as the initial block (given_init_evec), which B-orthonormalizes
 // them and iterates to the caller's tolerance, optionally with FP32 inner
 // solves (refine_bcg_precision) and CholQR (refine_w_orth).
+#include <cfloat>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -39,15 +40,17 @@ extern "C" GCGE_INT GCGE_HIP_SOLVER_SolveMixed(GCGE_SOLVER *solver, const GCGE_H
     GCGE_PARA *P = solver->para;
     const int nev = P->nev;
     GCGE_HIP_MIXED_OPTS o;
-    o.fp32_tol = (opts && opts->fp32_tol > 0) ? opts->fp32_tol : 1e-4;
+    o.fp32_tol = (opts && opts->fp32_tol > 0) ? opts->fp32_tol : 1e-7;
     o.fp32_max_it = (opts && opts->fp32_max_it > 0) ? opts->fp32_max_it : P->ev_max_it;
     o.fp64_max_it = (opts && opts->fp64_max_it > 0) ? opts->fp64_max_it : P->ev_max_it;
-    o.stagnation_window = opts ? opts->stagnation_window : 0;
+    // without opts: the configuration of the mixed path (FP32 phase with CholQR,
+    // stopped at its stagnation; refinement with CholQR and FP32 inner solves)
+    o.stagnation_window = opts ? opts->stagnation_window : 10;
     o.stagnation_ratio = (opts && opts->stagnation_ratio > 0) ? opts->stagnation_ratio : 0.7;
     o.fp32_w_orth = opts ? opts->fp32_w_orth : 1;
-    o.refine_w_orth = opts ? opts->refine_w_orth : 0;
-    o.refine_bcg_precision = opts ? opts->refine_bcg_precision : 0;
-    o.stagnation_start = opts ? opts->stagnation_start : 0.0;
+    o.refine_w_orth = opts ? opts->refine_w_orth : 1;
+    o.refine_bcg_precision = opts ? opts->refine_bcg_precision : 1;
+    o.stagnation_start = opts ? opts->stagnation_start : 1e-3;
     GCGE_HIP_MIXED_INFO inf;
     std::memset(&inf, 0, sizeof(inf));
 
@@ -77,6 +80,7 @@ extern "C" GCGE_INT GCGE_HIP_SOLVER_SolveMixed(GCGE_SOLVER *solver, const GCGE_H
         s32->para->stagnation_start = o.stagnation_start;
         s32->para->print_para = 0;
         s32->para->bcg_precision = 0;  // the phase is FP32 throughout
+        s32->para->basis_eps = FLT_EPSILON;
         if (o.fp32_w_orth) s32->para->w_orth_type = (char *)"cholqr";
     }
     s32->eval = (double *)calloc(nev, sizeof(double));

@@ -5,6 +5,7 @@
 // -gcge_cg_rate is parsed as a floating-point number (the reference uses
 // atoi, src/gcge_para.c:253, which turns every rate below 1 into 0).
 #include <cmath>
+#include <cfloat>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -86,6 +87,7 @@ extern "C" void GCGE_PARA_Create(GCGE_PARA **para)
     p->opt_allgatherv = 0;
     p->stagnation_start = 0.0;
     p->lock_converged = 1;
+    p->basis_eps = DBL_EPSILON;
     *para = p;
 }
 

@@ -20,7 +20,7 @@ constexpr uint32_t kMagicMV = 0x4d56434fu;   // multivector
 constexpr uint32_t kMagicCol = 0x4c4f4356u;  // column view / vector
 constexpr uint32_t kMagicMat = 0x54414d48u;  // HIP_MAT
 
-constexpr int kMaxCols = 64;      // per-launch column batch for per-column scalars
+constexpr int kMaxCols = 128;     // per-launch column batch for per-column scalars (2.5 KB of kernel arguments)
 constexpr int kMaxChunks = 512;   // split-n partial sums of reductions
 
 struct MV;

@@ -249,11 +249,11 @@ def synchronize():
     _check("synchronize")
 
 
-# GCGE_HIP_MIXED_OPTS of the two-phase forms (0 / 0.0 select the library's defaults:
-# fp32_tol 1e-4, the caller's ev_max_it, stagnation ratio 0.7)
-MIXED_DEFAULTS = dict(fp32_tol=0.0, fp32_max_it=0, fp64_max_it=0, stagnation_window=0, stagnation_ratio=0.0,
-                      fp32_w_orth=1, refine_w_orth=0, refine_bcg_precision=1, stagnation_start=0.0)
-FP32_REFINE_DEFAULTS = dict(MIXED_DEFAULTS, fp32_w_orth=0, refine_bcg_precision=0)
+# GCGE_HIP_MIXED_OPTS of the two-phase forms (0 selects the library's default for
+# fp32_tol, the iteration budgets -- the caller's ev_max_it -- and stagnation_ratio)
+MIXED_DEFAULTS = dict(fp32_tol=1e-7, fp32_max_it=0, fp64_max_it=0, stagnation_window=10, stagnation_ratio=0.7,
+                      fp32_w_orth=1, refine_w_orth=1, refine_bcg_precision=1, stagnation_start=1e-3)
+FP32_REFINE_DEFAULTS = dict(MIXED_DEFAULTS, fp32_w_orth=0, refine_w_orth=0, refine_bcg_precision=0)
 
 
 def solve(A, B, nev, X0=None, precision="fp64", mixed=None, seed=None, keep_evec=True, **params):

@@ -65,6 +65,7 @@ class GCGE_PARA(C.Structure):
         ("solve_seconds", c_double),
         ("stagnation_window", c_int), ("stagnation_ratio", c_double), ("stopped_on_stagnation", c_int),
         ("bcg_precision", c_int), ("stagnation_start", c_double), ("lock_converged", c_int),
+        ("basis_eps", c_double),
     ]
 
 

@@ -147,6 +147,51 @@ def fv_diffusion(kappa: np.ndarray, face_scale=None):
     return _finish(A)
 
 
+def fv_diffusion_jacobian(kappa: np.ndarray, A, face_scale=None):
+    """Jacobian of the stored values of A = fv_diffusion(kappa, face_scale)
+    with respect to the cell coefficients, as CSR over cells: for cell c,
+    jnz[jrowptr[c]:jrowptr[c+1]] are positions in A.data and jval the
+    derivatives dA.data[pos] / dkappa_c.  An interior face t = 2 k1 k2/(k1+k2)
+    has dt/dk1 = 2 k2^2/(k1+k2)^2; a wall face adds 2 k_i to the diagonal."""
+    shape = kappa.shape
+    d = len(shape)
+    n = kappa.size
+    idx = np.arange(n).reshape(shape)
+    keys = np.repeat(np.arange(n, dtype=np.int64), np.diff(A.indptr)) * n + A.indices.astype(np.int64)
+
+    def pos(r, c):
+        return np.searchsorted(keys, r.astype(np.int64) * n + c.astype(np.int64))
+
+    P, E, V = [], [], []
+    for ax in range(d):
+        s = 1.0 if face_scale is None else float(face_scale[ax])
+        lo = [slice(None)] * d
+        hi = [slice(None)] * d
+        lo[ax] = slice(0, shape[ax] - 1)
+        hi[ax] = slice(1, shape[ax])
+        k1 = kappa[tuple(lo)].ravel()
+        k2 = kappa[tuple(hi)].ravel()
+        i1 = idx[tuple(lo)].ravel()
+        i2 = idx[tuple(hi)].ravel()
+        den = (k1 + k2) ** 2
+        for cell, dt in ((i1, s * 2.0 * k2 * k2 / den), (i2, s * 2.0 * k1 * k1 / den)):
+            for (r, c, sign) in ((i1, i2, -1.0), (i2, i1, -1.0), (i1, i1, 1.0), (i2, i2, 1.0)):
+                P.append(cell)
+                E.append(pos(r, c))
+                V.append(sign * dt)
+        for side in (0, shape[ax] - 1):
+            sl = [slice(None)] * d
+            sl[ax] = slice(side, side + 1)
+            cells = idx[tuple(sl)].ravel()
+            P.append(cells)
+            E.append(pos(cells, cells))
+            V.append(np.full(cells.size, s * 2.0))
+    J = sp.csr_matrix((np.concatenate(V), (np.concatenate(P), np.concatenate(E))), shape=(n, A.nnz))
+    J.sum_duplicates()
+    J.sort_indices()
+    return J.indptr.astype(np.int32), J.indices.astype(np.int32), J.data.astype(np.float64)
+
+
 def vardiff3d(m: int = 160, sigma: float = 1.0, corr_len: float = 8.0, seed: int = 0):
     """3-D variable-coefficient diffusion on m^3 cells, kappa = exp(g)."""
     g = gaussian_field((m, m, m), corr_len, sigma, seed)

@@ -85,8 +85,8 @@ typedef struct {
     GCGE_INT refine_w_orth;   /* the same choice for the FP64 refinement                      */
     GCGE_INT refine_bcg_precision; /* 1: block CG of the refinement in FP32 as a correction
                                       of the FP64 residual (bcg_precision), 0: FP64           */
-    GCGE_DOUBLE stagnation_start;  /* stagnation is watched once max_res is below this
-                                      (0: from the first iteration)                           */
+    GCGE_DOUBLE stagnation_start;  /* stagnation is watched once max_res has fallen to this
+                                      fraction of its peak (0: from the first iteration)      */
 } GCGE_HIP_MIXED_OPTS;
 
 typedef struct {
@@ -98,11 +98,15 @@ typedef struct {
 } GCGE_HIP_MIXED_INFO;
 
 /* Solve with an FP32 phase (matrix values and multivectors in FP32, dense
- * Rayleigh-Ritz and the residual check in FP64) followed by FP64 refinement
- * sweeps of the same GCG iteration started from the FP32 eigenvectors
- * (given_init_evec).  solver must have been created by GCGE_HIP_Solver_Init
- * with FP64 evec; its para (nev, ev_tol, conv_type, ...) applies to the
- * refinement.  When given_init_evec is set, evec holds the start block. */
+ * Rayleigh-Ritz, Cholesky factors and the residual check in FP64) followed by
+ * FP64 refinement sweeps of the same GCG iteration started from the FP32
+ * eigenvectors (given_init_evec).  solver must have been created by
+ * GCGE_HIP_Solver_Init with FP64 evec; its para (nev, ev_tol, conv_type, ...)
+ * applies to the refinement.  When given_init_evec is set, evec holds the
+ * start block.  opts may be NULL: fp32_tol 1e-7, the caller's ev_max_it for
+ * both phases, stagnation window 10 / ratio 0.7 / start 1e-3, CholQR in both
+ * phases and FP32 inner solves in the refinement.  In opts, a zero fp32_tol,
+ * budget or stagnation_ratio selects the same default. */
 GCGE_INT GCGE_HIP_SOLVER_SolveMixed(GCGE_SOLVER *solver, const GCGE_HIP_MIXED_OPTS *opts,
                                     GCGE_HIP_MIXED_INFO *info);
 

@@ -130,13 +130,18 @@ typedef struct GCGE_PARA_ {
      * FP64-accumulated dots, and W = W0 + E is formed in FP64).  Needs the
      * MultiVecCreateByMatPrecision slot. */
     GCGE_INT bcg_precision;
-    /* input: stagnation is only watched once max_res is below this value
-     * (residuals of a random start first grow before they fall) */
+    /* input: when > 0, stagnation is only watched once max_res has fallen to
+     * this fraction of its largest value so far (residuals of a random start
+     * first grow before they fall) */
     GCGE_DOUBLE stagnation_start;
     /* input: 1 (the reference's behaviour) removes converged pairs from the
      * W and P blocks; 0 keeps every pair active until all have converged
      * (num_unlock stays nev until then) */
     GCGE_INT lock_converged;
+    /* input: unit roundoff of the basis storage (DBL_EPSILON for FP64,
+     * FLT_EPSILON for FP32); sets the drop and check tolerances of the
+     * "cholqr" W orthonormalization */
+    GCGE_DOUBLE basis_eps;
 } GCGE_PARA;
 
 void GCGE_PARA_Create(GCGE_PARA **para);

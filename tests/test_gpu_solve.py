@@ -187,6 +187,33 @@ def test_cholqr_w_orthonormalization(withB):
     assert np.all(ns_residual(A, B, r["evec"], r["eval"]) <= 1e-9 * 1.01)
 
 
+def test_config2_generalized_96cubed_closed_form():
+    """3-D 7-point Laplacian with the P1 mass matrix on 96^3 (n = 884,736),
+    lowest 50 pairs: eigenvalues from the Kronecker closed form."""
+    A, B = P.lap3d_mass(96)
+    nev = 50
+    X0 = P.random_x0(A.shape[0], nev, 9)
+    r = solve(A, B, nev, X0=X0, conv_type="NS", ev_tol=1e-8, ev_max_it=1000, keep_evec=False)
+    assert r["num_unlock"] == 0
+    np.testing.assert_allclose(r["eval"], P.lap3d_mass_eigs(96, nev), rtol=EV_RTOL)
+
+
+def test_config4_chung_lu_against_dense():
+    """Chung-Lu graph Laplacian + 1e-2 I (power-law rows, a degenerate lowest
+    spectrum from isolated vertices) at n = 3000: the device solve against
+    dense LAPACK and the oracle on the same start."""
+    A, _ = P.chung_lu(3000, mean_deg=16.0, seed=2)
+    nev = 12
+    X0 = P.random_x0(A.shape[0], nev, 10)
+    ref = np.linalg.eigvalsh(A.toarray())[:nev]
+    r = solve(A, None, nev, X0=X0, conv_type="NS", ev_tol=1e-9, ev_max_it=1000)
+    o = O.solve(A, None, nev, X0=X0, conv_type="NS", ev_tol=1e-9, ev_max_it=1000)
+    assert r["num_unlock"] == 0
+    np.testing.assert_allclose(r["eval"], ref, rtol=EV_RTOL)
+    np.testing.assert_allclose(r["eval"], o["eval"], rtol=EV_RTOL)
+    assert np.all(ns_residual(A, None, r["evec"], r["eval"]) <= 1e-9 * 1.01)
+
+
 def test_dirichlet_example_configuration():
     """example/csr_solver.c: A_5/M_5, nev = 10, Dirichlet rows zeroed in the
     random initial block."""
