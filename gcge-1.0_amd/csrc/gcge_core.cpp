@@ -301,23 +301,26 @@ namespace {
 //     (the block shrinks, as the reference deflates dependent vectors,
 //     gcge_orthogonal.c:1991-2001).
 // tau = 1e4 basis_eps bounds the condition of the factored Gram, so the
-// second pass restores orthonormality to working accuracy.  A round ends
-// with a check of V1^T B W; another round runs while its largest entry
-// exceeds 1e3 basis_eps (at most three rounds).  Columns with a B-norm at or
-// below orth_zero_tol are dropped first.
-void orth_factor_pass(void **V, int start, int *end, void *B, GCGE_OPS *ops, GCGE_PARA *para,
-                      GCGE_WORKSPACE *ws)
+// second pass restores orthonormality to working accuracy.  When a factor
+// pass took the eigendecomposition route or had an R_jj below 1e-2, the
+// round ends with a check of V1^T B W and another round runs while its
+// largest entry exceeds 1e3 basis_eps (at most three rounds).  Columns with
+// a B-norm at or below orth_zero_tol are dropped first.
+// One factorization pass; returns the smallest R_jj when the Cholesky route
+// was taken, 0 otherwise.
+double orth_factor_pass(void **V, int start, int *end, void *B, GCGE_OPS *ops, GCGE_PARA *para,
+                        GCGE_WORKSPACE *ws)
 {
     GCGE_INT ms[2], me[2];
     const double zero_tol = para->orth_para->orth_zero_tol;
     const double tau = 1e4 * para->basis_eps;
     int nw = *end - start;
-    if (nw <= 0) return;
+    if (nw <= 0) return 0.0;
     // drop vanishing columns (B-norms from the Gram diagonal)
     std::vector<double> S, Z, w, C, d;
     for (;;) {
         nw = *end - start;
-        if (nw <= 0) return;
+        if (nw <= 0) return 0.0;
         void **T = V;
         int tc = start;
         if (B) {
@@ -355,8 +358,10 @@ void orth_factor_pass(void **V, int start, int *end, void *B, GCGE_OPS *ops, GCG
         fprintf(stderr, "GCGE: CholQR needs dpotrf/dtrtri from the LAPACK library\n");
         abort();
     }
+    double rmin = 1.0;
     if (chol_ok)
-        for (int j = 0; j < nw && chol_ok; j++) chol_ok = 1.0 / S[(size_t)j * nw + j] > std::sqrt(tau);
+        for (int j = 0; j < nw; j++) rmin = std::min(rmin, 1.0 / S[(size_t)j * nw + j]);
+    chol_ok = chol_ok && rmin > std::sqrt(tau);
     if (chol_ok) {
         for (int j = 0; j < nw; j++)
             for (int i = 0; i <= j; i++) C[(size_t)j * nw + i] = d[i] * S[(size_t)j * nw + i];
@@ -388,6 +393,7 @@ void orth_factor_pass(void **V, int start, int *end, void *B, GCGE_OPS *ops, GCG
         ops->MultiVecCopy(ws->V_tmp, V, ms, me, ops);
     }
     *end = start + k;
+    return chol_ok ? rmin : 0.0;
 }
 
 void cholqr_orth(void **V, int start, int *end, void *B, GCGE_OPS *ops, GCGE_PARA *para, GCGE_WORKSPACE *ws)
@@ -403,8 +409,15 @@ void cholqr_orth(void **V, int start, int *end, void *B, GCGE_OPS *ops, GCGE_PAR
                 sub_orth(V, ms, me, B, ws->V_tmp, ws->subspace_dtmp, ops);
             }
         }
-        for (int pass = 0; pass < 2 && *end > start; pass++) orth_factor_pass(V, start, end, B, ops, para, ws);
+        double rmin = 1.0;
+        for (int pass = 0; pass < 2 && *end > start; pass++)
+            rmin = std::min(rmin, pass == 0 ? orth_factor_pass(V, start, end, B, ops, para, ws)
+                                            : std::max(orth_factor_pass(V, start, end, B, ops, para, ws), 0.0));
         if (start == 0 || *end <= start) return;
+        // W was orthogonal to V1 to working accuracy after the projections; a
+        // Cholesky factor with R_jj >= 1e-2 amplifies that overlap at most
+        // 100 times, so the check is needed only after a worse factorization
+        if (rmin >= 1e-2) return;
         // check V1^T B W
         const int nw = *end - start;
         void **T = V;

@@ -135,21 +135,14 @@ __global__ __launch_bounds__(kThreads) void k_gram_mfma(int64_t n, const T *__re
     const int64_t rb = (int64_t)chunk * rows_per_chunk;
     const int64_t re = min(n, rb + rows_per_chunk);
 
+    // Columns past p (q) are clamped to the last one: they feed only result
+    // rows (columns) >= p (q), which are never stored, so no lane branches.
     const T *vc[WP];
     const T *wc[WQ];
-    bool vok[WP], wok[WQ];
 #pragma unroll
-    for (int x = 0; x < WP; x++) {
-        const int c = i0 + x * 16 + m;
-        vok[x] = c < p;
-        vc[x] = V + (int64_t)(vok[x] ? c : 0) * ldv;
-    }
+    for (int x = 0; x < WP; x++) vc[x] = V + (int64_t)min(i0 + x * 16 + m, p - 1) * ldv;
 #pragma unroll
-    for (int y = 0; y < WQ; y++) {
-        const int c = j0 + y * 16 + m;
-        wok[y] = c < q;
-        wc[y] = W + (int64_t)(wok[y] ? c : 0) * ldw;
-    }
+    for (int y = 0; y < WQ; y++) wc[y] = W + (int64_t)min(j0 + y * 16 + m, q - 1) * ldw;
 
     acc_t acc[WP][WQ];
 #pragma unroll
@@ -157,27 +150,54 @@ __global__ __launch_bounds__(kThreads) void k_gram_mfma(int64_t n, const T *__re
 #pragma unroll
         for (int y = 0; y < WQ; y++) acc[x][y] = acc_t{};
 
-    for (int64_t s = rb + (int64_t)wk * S; s < re; s += (int64_t)GK * S) {
+    // full supersteps (every row below re): unguarded vector loads, the next
+    // superstep's operands loaded before this one's MFMAs are issued
+    using vec_t = typename MT<T>::vec_t;
+    const int64_t step = (int64_t)GK * S;
+    const int64_t full_end = rb + ((re - rb) / S) * S;
+    int64_t s = rb + (int64_t)wk * S;
+    if (s < full_end) {
+        // supersteps in pairs with two operand buffers (loads of one in
+        // flight while the MFMAs of the other are issued)
+        vec_t a0[WP], b0[WQ], a1[WP], b1[WQ];
+        auto load = [&](int64_t ss, vec_t (&aa)[WP], vec_t (&bb)[WQ]) {
+#pragma unroll
+            for (int x = 0; x < WP; x++) aa[x] = *reinterpret_cast<const vec_t *>(vc[x] + ss + (int64_t)kk * R);
+#pragma unroll
+            for (int y = 0; y < WQ; y++) bb[y] = *reinterpret_cast<const vec_t *>(wc[y] + ss + (int64_t)kk * R);
+        };
+        auto mma = [&](const vec_t (&aa)[WP], const vec_t (&bb)[WQ]) {
+#pragma unroll
+            for (int t = 0; t < R; t++)
+#pragma unroll
+                for (int x = 0; x < WP; x++)
+#pragma unroll
+                    for (int y = 0; y < WQ; y++) acc[x][y] = MT<T>::mma(aa[x][t], bb[y][t], acc[x][y]);
+        };
+        // the last full superstep of this wavefront
+        const int64_t last_full = s + ((full_end - 1 - s) / step) * step;
+        load(s, a0, b0);
+        for (; s + step < full_end; s += 2 * step) {
+            load(s + step, a1, b1);
+            mma(a0, b0);
+            // unconditional (clamped to a full superstep) so the wait before
+            // the next MFMAs covers only the older loads
+            load(min(s + 2 * step, last_full), a0, b0);
+            mma(a1, b1);
+        }
+        if (s < full_end) {
+            mma(a0, b0);
+            s += step;
+        }
+    }
+    // the partial superstep at the end of the chunk (rows past re read as 0)
+    if (s < re) {
         const int64_t row = s + (int64_t)kk * R;
         T a[WP][R], b[WQ][R];
 #pragma unroll
-        for (int x = 0; x < WP; x++) {
-            if (vok[x]) {
-                load_rows<T>(vc[x], row, re, a[x]);
-            } else {
+        for (int x = 0; x < WP; x++) load_rows<T>(vc[x], row, re, a[x]);
 #pragma unroll
-                for (int t = 0; t < R; t++) a[x][t] = T(0);
-            }
-        }
-#pragma unroll
-        for (int y = 0; y < WQ; y++) {
-            if (wok[y]) {
-                load_rows<T>(wc[y], row, re, b[y]);
-            } else {
-#pragma unroll
-                for (int t = 0; t < R; t++) b[y][t] = T(0);
-            }
-        }
+        for (int y = 0; y < WQ; y++) load_rows<T>(wc[y], row, re, b[y]);
 #pragma unroll
         for (int t = 0; t < R; t++)
 #pragma unroll
@@ -305,8 +325,11 @@ __global__ __launch_bounds__(kThreads) void k_lincomb_mfma(int64_t n, const T *_
     const int m = lane & 15, kk = lane >> 4;
     const int64_t rb = ((int64_t)blockIdx.x * 4 + wave) * 32;
     if (rb >= n) return;  // no workgroup barrier in this kernel
-    const int64_t ra0 = rb + m, ra1 = rb + 16 + m;
-    const bool ok0 = ra0 < n, ok1 = ra1 < n;
+    // Rows past n are clamped to the last row: they feed only result rows
+    // >= n, which are never stored.  nqb is a multiple of RN (Cp is padded
+    // with zero blocks), and only the last k block can be partial.
+    const int64_t ra0 = min(rb + m, n - 1), ra1 = min(rb + 16 + m, n - 1);
+    const int nkb_full = p / 4;
     const T al = (T)alpha, be = (T)beta;
     for (int jb0 = 0; jb0 < nqb; jb0 += RN) {
         acc_t acc0[RN], acc1[RN];
@@ -315,27 +338,59 @@ __global__ __launch_bounds__(kThreads) void k_lincomb_mfma(int64_t n, const T *_
             acc0[r] = acc_t{};
             acc1[r] = acc_t{};
         }
-#pragma unroll 2
-        for (int kb = 0; kb < nkb; kb++) {
-            const int k = kb * 4 + kk;
+        if (nkb_full > 0) {
+            // k blocks in pairs with two operand buffers: the loads of one
+            // block are in flight while the MFMAs of the other are issued
+            const T *xk = X + (int64_t)kk * ldx;
+            const T *cb = Cp + (int64_t)jb0 * 64 + lane;
+            const int64_t xs = (int64_t)4 * ldx, cs = (int64_t)nqb * 64;
+            T a0[2], a1[2], b0[RN], b1[RN];
+            auto load = [&](int kb, T (&aa)[2], T (&bb)[RN]) {
+                const T *x = xk + (int64_t)kb * xs;
+                const T *c = cb + (int64_t)kb * cs;
+                aa[0] = x[ra0];
+                aa[1] = x[ra1];
+#pragma unroll
+                for (int r = 0; r < RN; r++) bb[r] = c[r * 64];
+            };
+            auto mma = [&](const T (&aa)[2], const T (&bb)[RN]) {
+#pragma unroll
+                for (int r = 0; r < RN; r++) {
+                    acc0[r] = MT<T>::mma(aa[0], bb[r], acc0[r]);
+                    acc1[r] = MT<T>::mma(aa[1], bb[r], acc1[r]);
+                }
+            };
+            load(0, a0, b0);
+            int kb = 0;
+            for (; kb + 1 < nkb_full; kb += 2) {
+                load(kb + 1, a1, b1);
+                mma(a0, b0);
+                // unconditional (clamped) so the wait before the next MFMAs
+                // covers only the older loads
+                load(min(kb + 2, nkb_full - 1), a0, b0);
+                mma(a1, b1);
+            }
+            if (kb < nkb_full) mma(a0, b0);
+        }
+        if (nkb_full < nkb) {
+            // partial last k block: columns past p read as 0
+            const int k = nkb_full * 4 + kk;
             const bool kok = k < p;
             const T *xk = X + (int64_t)(kok ? k : 0) * ldx;
-            const T a0 = (kok && ok0) ? xk[ra0] : T(0);
-            const T a1 = (kok && ok1) ? xk[ra1] : T(0);
-            const T *cb = Cp + ((int64_t)kb * nqb + jb0) * 64 + lane;
+            const T a0 = kok ? xk[ra0] : T(0);
+            const T a1 = kok ? xk[ra1] : T(0);
+            const T *cb = Cp + ((int64_t)nkb_full * nqb + jb0) * 64 + lane;
 #pragma unroll
             for (int r = 0; r < RN; r++) {
-                if (jb0 + r < nqb) {
-                    const T b = cb[r * 64];
-                    acc0[r] = MT<T>::mma(a0, b, acc0[r]);
-                    acc1[r] = MT<T>::mma(a1, b, acc1[r]);
-                }
+                const T bb = cb[r * 64];
+                acc0[r] = MT<T>::mma(a0, bb, acc0[r]);
+                acc1[r] = MT<T>::mma(a1, bb, acc1[r]);
             }
         }
 #pragma unroll
         for (int r = 0; r < RN; r++) {
             const int j = (jb0 + r) * 16 + m;
-            if (jb0 + r >= nqb || j >= q) continue;
+            if (j >= q) continue;
             T *yj = Y + (int64_t)j * ldy;
 #pragma unroll
             for (int h = 0; h < 2; h++) {
@@ -599,7 +654,9 @@ bool lincomb(int prec, int64_t n, const void *X, int64_t ldx, int p, void *Y, in
         else launch_lincomb_thin<float>(n, X, ldx, p, Y, ldy, q, cf, alpha, beta);
         return check(hipGetLastError(), "k_lincomb_thin");
     }
-    const int nkb = (p + 3) / 4, nqb = (q + 15) / 16;
+    const int nkb = (p + 3) / 4, nqb0 = (q + 15) / 16;
+    const int rn = nqb0 <= 1 ? 1 : nqb0 <= 2 ? 2 : nqb0 <= 4 ? 4 : 8;
+    const int nqb = ((nqb0 + rn - 1) / rn) * rn;  // zero blocks pad the last column group
     const size_t ncp = (size_t)nkb * nqb * 64;
     const void *dC = nullptr;
     if (prec == GCGE_HIP_FP64) {
@@ -630,14 +687,14 @@ bool lincomb(int prec, int64_t n, const void *X, int64_t ldx, int p, void *Y, in
     k_lincomb_mfma<T_, RN_><<<grid, kThreads, 0, ctx().stream>>>(n, (const T_ *)X, ldx, p, (T_ *)Y, ldy, q,    \
                                                                  (const T_ *)dC, nkb, nqb, alpha, beta)
         if (prec == GCGE_HIP_FP64) {
-            if (nqb <= 1) GH_LC(double, 1);
-            else if (nqb <= 2) GH_LC(double, 2);
-            else if (nqb <= 4) GH_LC(double, 4);
+            if (rn == 1) GH_LC(double, 1);
+            else if (rn == 2) GH_LC(double, 2);
+            else if (rn == 4) GH_LC(double, 4);
             else GH_LC(double, 8);
         } else {
-            if (nqb <= 1) GH_LC(float, 1);
-            else if (nqb <= 2) GH_LC(float, 2);
-            else if (nqb <= 4) GH_LC(float, 4);
+            if (rn == 1) GH_LC(float, 1);
+            else if (rn == 2) GH_LC(float, 2);
+            else if (rn == 4) GH_LC(float, 4);
             else GH_LC(float, 8);
         }
 #undef GH_LC
