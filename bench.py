@@ -72,23 +72,52 @@ def kernel_delta(after, before):
     return out
 
 
-def roofline(stats, prec_peak_tflops):
-    """Dominant kernel class (largest device time) and its achieved rate."""
+# profiler class -> the kernel template whose instantiations rocprofv3 lists
+ROCPROF_KERNEL = {
+    "spmm_f64": "k_spmm<double, false>", "spmm_f32": "k_spmm<float, false>",
+    "spmm_dot_f64": "k_spmm<double, true>", "spmm_dot_f32": "k_spmm<float, true>",
+    "axpby": "k_axpby<...>", "dots": "k_dots<...>",
+    "cg_update_f64": "k_cg_update<double>", "cg_update_f32": "k_cg_update<float>",
+    "gram_mfma_f64": "k_gram_mfma<double, ...>", "gram_mfma_f32": "k_gram_mfma<float, ...>",
+    "gram_thin": "k_gram_thin<...>", "lincomb_mfma_f64": "k_lincomb_mfma<double, ...>",
+    "lincomb_mfma_f32": "k_lincomb_mfma<float, ...>", "lincomb_thin": "k_lincomb_thin<...>",
+    "residual": "k_residual<...>", "copy": "k_copy<...>", "swap": "k_swap<...>",
+}
+
+
+def load_traffic():
+    """HBM bytes per launch measured with rocprofv3 --pmc (profiles/traffic.json)."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def roofline(stats):
+    """Dominant kernel class (largest device time) and its achieved rate:
+    algorithmic bytes (or flops, for the MFMA tiles) per launch / average
+    launch time, from HIP events on the backend stream."""
     if not stats:
         return None, None
     name = max(stats, key=lambda k: stats[k]["ms"])
     s = stats[name]
     if s["ms"] <= 0:
         return name, None
-    if name in ("gram", "lincomb"):
+    cnt = max(1, s["count"])
+    tr = load_traffic().get(name)
+    common = dict(kernel=ROCPROF_KERNEL.get(name, name), launches=s["count"],
+                  avg_ms=round(s["ms"] / cnt, 5), bytes_per_launch=round(s["bytes"] / cnt),
+                  traffic=(tr or {}).get("bytes_per_launch"))
+    if "mfma" in name:
+        peak = FP64_PEAK_TFLOPS if name.endswith("f64") else FP32_PEAK_TFLOPS
         ach = s["flops"] / (s["ms"] * 1e-3) / 1e12
-        return name, dict(bound="mfma", achieved=round(ach, 3), peak=prec_peak_tflops, unit="TFLOP/s",
-                          frac=round(ach / prec_peak_tflops, 4), traffic=None)
+        return name, dict(bound="mfma", achieved=round(ach, 3), peak=peak, unit="TFLOP/s",
+                          frac=round(ach / peak, 4), flops_per_launch=round(s["flops"] / cnt), **common)
     ach = s["bytes"] / (s["ms"] * 1e-3) / 1e9
     return name, dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                      frac=round(ach / HBM_PEAK_GBS, 4), traffic=None,
-                      launches=s["count"], avg_ms=round(s["ms"] / max(1, s["count"]), 4),
-                      bytes_per_launch=s["bytes"] / max(1, s["count"]))
+                      frac=round(ach / HBM_PEAK_GBS, 4), **common)
 
 
 def cpu_baseline(args, n_full, nnz_full, fp64_iters, nev):
@@ -136,6 +165,23 @@ def cpu_baseline(args, n_full, nnz_full, fp64_iters, nev):
                 seconds_measured=round(secs, 3), projected_seconds=round(projected, 2))
 
 
+def copy_probe():
+    from gcge_hip import Matrix, MultiVec, Ops, capi
+    from gcge_hip import problems as P
+    A, _ = P.lap2d(2024)                  # n = 4,096,576 rows
+    mA = Matrix(A)
+    k = 16
+    x = MultiVec(mA, k)
+    y = MultiVec(mA, k)
+    ops = Ops()
+    for _ in range(3):
+        ops.copy(x, y)
+    capi.load().GCGE_HIP_Synchronize()
+    n = A.shape[0]
+    print(json.dumps({"copy_probe": {"kernel": "k_copy<double, double>", "rows": n, "columns": k,
+                                     "read_bytes": 8 * n * k, "write_bytes": 8 * n * k, "launches": 3}}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -160,7 +206,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample-size", type=int, default=24)
     ap.add_argument("--cpu-sample-iters", type=int, default=3)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run k_copy<double, double> over 16 columns of n = 4,096,000 rows three times "
+                         "(1 GiB moved per launch), the reference of known size for rocprofv3 --pmc byte counters")
     args = ap.parse_args()
+
+    if args.copy_probe:
+        copy_probe()
+        return
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
@@ -248,7 +301,7 @@ def main():
         except Exception as e:  # the baseline must not hide the GPU result
             log(f"cpu baseline failed: {e}")
 
-    dom, roof = roofline(mixed_stats, FP32_PEAK_TFLOPS)
+    dom, roof = roofline(mixed_stats)
     line = {
         "metric": "Eigenpairs/sec to 1e-8 rel. residual, mixed vs FP64 GCGE, 1 MI355X",
         "value": round(value, 4),
@@ -280,7 +333,7 @@ def main():
         "cpu_baseline": cpu,
     }
     if fp64:
-        d64, roof64 = roofline(fp64["stats"], FP64_PEAK_TFLOPS)
+        d64, roof64 = roofline(fp64["stats"])
         line["fp64"] = dict(eigenpairs_per_s=round(nev / fp64["seconds"], 4), seconds=round(fp64["seconds"], 3),
                             iters=fp64["iters"], converged=fp64["unconverged"] == 0,
                             roofline_kernel=d64, roofline=roof64,

@@ -560,7 +560,7 @@ void bcg(void *A, void **RHS, void **V, int x_start, int x_length, GCGE_OPS *ops
     const int max_it = para->cg_max_it;
     const double rate = para->cg_rate;
     GCGE_INT ms[2], me[2];
-    std::vector<double> a(x_length), b(x_length), alpha(x_length), rr(x_length);
+    std::vector<double> b(x_length), alpha(x_length), rr(x_length);
 
     if (!x_is_zero) {
         set2(ms, x_start, 0);
@@ -576,7 +576,11 @@ void bcg(void *A, void **RHS, void **V, int x_start, int x_length, GCGE_OPS *ops
     for (int idx = 0; idx < x_length; idx++) unlock[idx] = idx;
     int num_unlock = x_length;
     for (int idx = 0; idx < x_length; idx++) error[idx] = std::sqrt(rho2[idx]);
-    int niter = 1;
+    // The solution update x += alpha p of a step is deferred into the search
+    // direction update of the next step (p is read there before it is
+    // overwritten); every element sees the reference's roundings in the
+    // reference's order (gcge_cg.c:197-286).
+    int niter = 1, na_prev = 0;
     while (num_unlock > 0 && niter < max_it) {
         const int na = num_unlock;  // active columns are unlock[0..na) = 0..na-1
         set2(ms, 0, 0);
@@ -584,18 +588,16 @@ void bcg(void *A, void **RHS, void **V, int x_start, int x_length, GCGE_OPS *ops
         if (niter == 1) {
             ops->MultiVecAxpby(1.0, RHS, 0.0, CG_P, ms, me, ops);
         } else {
-            for (int id = 0; id < na; id++) {
-                a[id] = 1.0;
-                b[id] = rho2[id] / rho1[id];
-            }
-            ops->MultiVecAxpbyDiag(a.data(), RHS, b.data(), CG_P, ms, me, ops);
+            for (int id = 0; id < na; id++) b[id] = rho2[id] / rho1[id];
+            ops->BlockCGSearchUpdate(V, x_start, CG_P, RHS, na_prev, na, alpha.data(), b.data(), ops);
         }
         ops->MatDotMultiVecInnerProdDiag(A, CG_P, CG_W, ms, me, ptw, ops);
         for (int id = 0; id < na; id++) {
             alpha[id] = rho2[id] / ptw[id];
             rho1[id] = rho2[id];
         }
-        ops->BlockCGUpdate(V, x_start, CG_P, CG_W, RHS, na, alpha.data(), rr.data(), ops);
+        ops->BlockCGResidualUpdate(CG_W, RHS, na, alpha.data(), rr.data(), ops);
+        na_prev = na;
         for (int id = 0; id < na; id++) rho2[id] = rr[id];
         for (int idx = 0; idx < x_length; idx++) last_error[idx] = std::sqrt(std::fabs(rho2[idx]));
         num_unlock = 0;
@@ -607,6 +609,7 @@ void bcg(void *A, void **RHS, void **V, int x_start, int x_length, GCGE_OPS *ops
         }
         niter++;
     }
+    if (na_prev > 0) ops->BlockCGSearchUpdate(V, x_start, CG_P, RHS, na_prev, 0, alpha.data(), b.data(), ops);
     para->bcg_calls++;
     para->bcg_iters_total += niter - 1;
 }
@@ -1213,6 +1216,8 @@ extern "C" GCGE_INT GCGE_OPS_Setup(GCGE_OPS *ops)
         {(const void *)ops->MultiVecResidualNorms, "MultiVecResidualNorms"},
         {(const void *)ops->MultiVecGather, "MultiVecGather"},
         {(const void *)ops->MultiVecCopy, "MultiVecCopy"},
+        {(const void *)ops->BlockCGResidualUpdate, "BlockCGResidualUpdate"},
+        {(const void *)ops->BlockCGSearchUpdate, "BlockCGSearchUpdate"},
     };
     for (auto &r : req) {
         if (!r.p) {

@@ -158,8 +158,12 @@ static void prof_resolve()
 
 const char *kclass_name(int k)
 {
-    static const char *names[K_NUM] = {"spmm", "spmm_dot", "axpby", "dots", "cg_update", "gram",
-                                       "lincomb", "residual", "copy", "swap", "other"};
+    // one class per kernel family and storage precision (k_spmm<T, DOT>,
+    // k_cg_update<T>, k_gram_mfma<T, ...>, k_lincomb_mfma<T, ...>, ...)
+    static const char *names[K_NUM] = {"spmm_f64", "spmm_f32", "spmm_dot_f64", "spmm_dot_f32", "axpby", "dots",
+                                       "cg_update_f64", "cg_update_f32", "gram_mfma_f64", "gram_mfma_f32",
+                                       "gram_thin", "lincomb_mfma_f64", "lincomb_mfma_f32", "lincomb_thin",
+                                       "residual", "copy", "swap", "other"};
     return (k >= 0 && k < K_NUM) ? names[k] : "?";
 }
 
@@ -945,6 +949,34 @@ void s_BlockCGUpdate(void **x, GCGE_INT sx, void **p, void **w, void **r, GCGE_I
         for (int j = 0; j < k; j++) rr[j] = NAN;
 }
 
+void s_BlockCGResidualUpdate(void **w, void **r, GCGE_INT k, const GCGE_DOUBLE *alpha, GCGE_DOUBLE *rr, GCGE_OPS *)
+{
+    MV *mw = as_mv(w), *mr = as_mv(r);
+    if (!range_ok(mw, 0, k, "BlockCGResidualUpdate") || !range_ok(mr, 0, k, "BlockCGResidualUpdate") ||
+        !same_rows(mw, mr, "BlockCGResidualUpdate") || !same_prec(mw, mr, "BlockCGResidualUpdate"))
+        return;
+    if (!cg_resid(mr->prec, mr->n, mw->data, mw->ld, mr->data, mr->ld, k, alpha, rr))
+        for (int j = 0; j < k; j++) rr[j] = NAN;
+}
+
+void s_BlockCGSearchUpdate(void **x, GCGE_INT sx, void **p, void **r, GCGE_INT kx, GCGE_INT kp,
+                           const GCGE_DOUBLE *alpha, const GCGE_DOUBLE *beta, GCGE_OPS *)
+{
+    MV *mx = as_mv(x), *mp = as_mv(p), *mr = as_mv(r);
+    if (kp < 0 || kx < 0) {
+        latch(172, "BlockCGSearchUpdate: negative column count (kx %d, kp %d)", kx, kp);
+        return;
+    }
+    const int k = std::max(kx, kp);
+    if (!range_ok(mx, sx, sx + kx, "BlockCGSearchUpdate") || !range_ok(mp, 0, k, "BlockCGSearchUpdate") ||
+        !range_ok(mr, 0, kp, "BlockCGSearchUpdate") || !same_rows(mx, mp, "BlockCGSearchUpdate") ||
+        !same_rows(mx, mr, "BlockCGSearchUpdate") || !same_prec(mx, mp, "BlockCGSearchUpdate") ||
+        !same_prec(mx, mr, "BlockCGSearchUpdate"))
+        return;
+    launch_cg_search(mx->prec, mx->n, col_ptr(mx, sx), mx->ld, mp->data, mp->ld, mr->data, mr->ld, kx, kp, alpha,
+                     beta);
+}
+
 void s_MultiVecResidualNorms(void *A, void *B, void **X, const GCGE_INT *idx, GCGE_INT k, const GCGE_DOUBLE *lambda,
                              GCGE_DOUBLE *rnorm, GCGE_DOUBLE *xnorm, GCGE_DOUBLE *bxnorm, GCGE_OPS *)
 {
@@ -1056,6 +1088,8 @@ extern "C" void GCGE_HIP_SetOps(GCGE_OPS *ops)
     ops->MultiVecLinearCombAxpby = s_MultiVecLinearCombAxpby;
     ops->MatDotMultiVecInnerProdDiag = s_MatDotMultiVecInnerProdDiag;
     ops->BlockCGUpdate = s_BlockCGUpdate;
+    ops->BlockCGResidualUpdate = s_BlockCGResidualUpdate;
+    ops->BlockCGSearchUpdate = s_BlockCGSearchUpdate;
     ops->MultiVecResidualNorms = s_MultiVecResidualNorms;
     ops->MultiVecGather = s_MultiVecGather;
     ops->MultiVecCopy = s_MultiVecCopy;

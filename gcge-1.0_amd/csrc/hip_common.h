@@ -94,8 +94,11 @@ bool sync();                           // stream synchronize + error check
 // ---- kernel profiler (opt-in): HIP events around each launch of a kernel
 // class on the backend stream, with the algorithmic bytes and flops of the
 // launch; resolved at stream synchronisation points.
-enum KClass { K_SPMM = 0, K_SPMM_DOT, K_AXPBY, K_DOTS, K_CG_UPDATE, K_GRAM, K_LINCOMB, K_RESIDUAL, K_COPY,
-              K_SWAP, K_OTHER, K_NUM };
+enum KClass {
+    K_SPMM_F64 = 0, K_SPMM_F32, K_SPMM_DOT_F64, K_SPMM_DOT_F32, K_AXPBY, K_DOTS, K_CG_UPDATE_F64, K_CG_UPDATE_F32,
+    K_GRAM_MFMA_F64, K_GRAM_MFMA_F32, K_GRAM_THIN, K_LINCOMB_MFMA_F64, K_LINCOMB_MFMA_F32, K_LINCOMB_THIN,
+    K_RESIDUAL, K_COPY, K_SWAP, K_OTHER, K_NUM
+};
 struct ProfScope {
     int kind;
     bool on;
@@ -154,6 +157,12 @@ bool residual_norms(int prec, HIP_MAT *A, HIP_MAT *B, const void *X, int64_t ldx
 // x_j += alpha_j p_j ; r_j -= alpha_j w_j ; rr_j = <r_j, r_j>
 bool cg_update(int prec, int64_t n, void *x, int64_t ldx, const void *p, int64_t ldp, const void *w,
                int64_t ldw, void *r, int64_t ldr, int k, const double *alpha, double *rr);
+// r_j -= alpha_j w_j ; rr_j = <r_j, r_j>
+bool cg_resid(int prec, int64_t n, const void *w, int64_t ldw, void *r, int64_t ldr, int k, const double *alpha,
+              double *rr);
+// x_j += alpha_j p_j (j < kx), then p_j = r_j + beta_j p_j (j < kp)
+void launch_cg_search(int prec, int64_t n, void *x, int64_t ldx, void *p, int64_t ldp, const void *r, int64_t ldr,
+                      int kx, int kp, const double *alpha, const double *beta);
 // G (p x q, host, column-major ldg) = V^T W
 bool gram(int prec, int64_t n, const void *V, int64_t ldv, int p, const void *W, int64_t ldw, int q,
           double *G, int ldg, bool upper_only);

@@ -312,6 +312,58 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(int64_t n, T *__restrict__
     if (threadIdx.x == 0) partial[(int64_t)j * gridDim.x + blockIdx.x] = bs;
 }
 
+// ---------------------------------------------------------------- split CG steps
+// r -= alpha w (VecAxpby(-alpha, w, 1.0, r)); partial r.r  (gcge_cg.c:250-276)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_resid(int64_t n, const T *__restrict__ w, int64_t ldw,
+                                                     T *__restrict__ r, int64_t ldr, ColScalars s,
+                                                     double *__restrict__ partial)
+{
+    __shared__ double sh[4];
+    const int j = blockIdx.y;
+    const T al = (T)s.a[j];
+    const T *wj = w + (int64_t)j * ldw;
+    T *rj = r + (int64_t)j * ldr;
+    double acc = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+        const T rn = axpby_el<T>(-al, wj[i], T(1), rj[i]);
+        rj[i] = rn;
+        acc = fma((double)rn, (double)rn, acc);
+    }
+    const double bs = block_sum(acc, sh);
+    if (threadIdx.x == 0) partial[(int64_t)j * gridDim.x + blockIdx.x] = bs;
+}
+
+// x += alpha p (VecAxpby(alpha, p, 1.0, x)) for the columns j < kx; then
+// p = r + beta p (MultiVecAxpbyDiag(1, r, beta, p)) for the columns j < kp
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_search(int64_t n, T *__restrict__ x, int64_t ldx,
+                                                      T *__restrict__ p, int64_t ldp, const T *__restrict__ r,
+                                                      int64_t ldr, int kx, int kp, ColScalars s)
+{
+    const int j = blockIdx.y;
+    const T al = (T)s.a[j], be = (T)s.b[j];
+    T *xj = x + (int64_t)j * ldx;
+    T *pj = p + (int64_t)j * ldp;
+    const T *rj = r + (int64_t)j * ldr;
+    const bool dx = j < kx, dp = j < kp;
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    if (dx && dp) {
+        for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+            const T pv = pj[i];
+            xj[i] = axpby_el<T>(al, pv, T(1), xj[i]);
+            pj[i] = axpby_el<T>(T(1), rj[i], be, pv);
+        }
+    } else if (dx) {
+        for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
+            xj[i] = axpby_el<T>(al, pj[i], T(1), xj[i]);
+    } else if (dp) {
+        for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
+            pj[i] = axpby_el<T>(T(1), rj[i], be, pj[i]);
+    }
+}
+
 // ---------------------------------------------------------------- gradient
 // g_e = sum_i w_i x_i[row_e] x_i[col_e], one thread per stored entry
 template <typename T>
@@ -497,7 +549,9 @@ static bool spmm_t(HIP_MAT *A, const void *X, int64_t ldx, void *Y, int64_t ldy,
         }
         {
             const double bytes = (sizeof(T) + 4.0) * A->nnz + 4.0 * (A->n + 1) + 2.0 * kc * (double)A->n * sizeof(T);
-            ProfScope ps(DOT ? K_SPMM_DOT : K_SPMM, bytes, 2.0 * kc * (double)A->nnz);
+            const bool f64 = sizeof(T) == 8;
+            ProfScope ps(DOT ? (f64 ? K_SPMM_DOT_F64 : K_SPMM_DOT_F32) : (f64 ? K_SPMM_F64 : K_SPMM_F32), bytes,
+                         2.0 * kc * (double)A->nnz);
             k_spmm<T, DOT><<<nblk, kSpmmRows, 0, ctx().stream>>>(A->n, A->rowptr, A->col, val, Xc, ldx, Yc, ldy,
                                                                   kc, g, partial);
         }
@@ -579,7 +633,8 @@ bool cg_update(int prec, int64_t n, void *x, int64_t ldx, const void *p, int64_t
         if (!partial) return false;
         dim3 grid(nb, kc);
         {
-            ProfScope ps(K_CG_UPDATE, 6.0 * kc * (double)n * elem_size(prec), 6.0 * kc * (double)n);
+            ProfScope ps(prec == GCGE_HIP_FP64 ? K_CG_UPDATE_F64 : K_CG_UPDATE_F32,
+                         6.0 * kc * (double)n * elem_size(prec), 6.0 * kc * (double)n);
             if (prec == GCGE_HIP_FP64)
                 k_cg_update<double><<<grid, kBlock, 0, ctx().stream>>>(
                     n, (double *)x + (int64_t)c0 * ldx, ldx, (const double *)p + (int64_t)c0 * ldp, ldp,
@@ -593,6 +648,64 @@ bool cg_update(int prec, int64_t n, void *x, int64_t ldx, const void *p, int64_t
         if (!reduce_to_host(partial, kc, nb, rr + c0)) return false;
     }
     return true;
+}
+
+bool cg_resid(int prec, int64_t n, const void *w, int64_t ldw, void *r, int64_t ldr, int k, const double *alpha,
+              double *rr)
+{
+    if (k <= 0) return true;
+    if (failed()) return false;
+    const int nb = grid_1d(n, 256);
+    for (int c0 = 0; c0 < k; c0 += kMaxCols) {
+        const int kc = std::min(kMaxCols, k - c0);
+        ColScalars s = make_scalars(kc, alpha + c0, nullptr, nullptr);
+        double *partial = (double *)scratch(sizeof(double) * ((size_t)kc * nb + kc));
+        if (!partial) return false;
+        dim3 grid(nb, kc);
+        {
+            ProfScope ps(prec == GCGE_HIP_FP64 ? K_CG_UPDATE_F64 : K_CG_UPDATE_F32,
+                         3.0 * kc * (double)n * elem_size(prec), 4.0 * kc * (double)n);
+            if (prec == GCGE_HIP_FP64)
+                k_cg_resid<double><<<grid, kBlock, 0, ctx().stream>>>(
+                    n, (const double *)w + (int64_t)c0 * ldw, ldw, (double *)r + (int64_t)c0 * ldr, ldr, s, partial);
+            else
+                k_cg_resid<float><<<grid, kBlock, 0, ctx().stream>>>(
+                    n, (const float *)w + (int64_t)c0 * ldw, ldw, (float *)r + (int64_t)c0 * ldr, ldr, s, partial);
+        }
+        if (!check(hipGetLastError(), "k_cg_resid")) return false;
+        if (!reduce_to_host(partial, kc, nb, rr + c0)) return false;
+    }
+    return true;
+}
+
+void launch_cg_search(int prec, int64_t n, void *x, int64_t ldx, void *p, int64_t ldp, const void *r, int64_t ldr,
+                      int kx, int kp, const double *alpha, const double *beta)
+{
+    const int k = std::max(kx, kp);
+    if (k <= 0 || failed()) return;
+    const int nb = grid_1d(n, 1024);
+    for (int c0 = 0; c0 < k; c0 += kMaxCols) {
+        const int kc = std::min(kMaxCols, k - c0);
+        const int kxc = std::max(0, std::min(kc, kx - c0)), kpc = std::max(0, std::min(kc, kp - c0));
+        std::vector<double> aa(kc, 0.0), bb(kc, 0.0);
+        for (int j = 0; j < kxc; j++) aa[j] = alpha[c0 + j];
+        for (int j = 0; j < kpc; j++) bb[j] = beta[c0 + j];
+        ColScalars s = make_scalars(kc, aa.data(), bb.data(), nullptr);
+        dim3 grid(nb, kc);
+        {
+            ProfScope ps(K_AXPBY, (3.0 * kxc + 3.0 * kpc - 1.0 * std::min(kxc, kpc)) * (double)n * elem_size(prec),
+                         0.0);
+            if (prec == GCGE_HIP_FP64)
+                k_cg_search<double><<<grid, kBlock, 0, ctx().stream>>>(
+                    n, (double *)x + (int64_t)c0 * ldx, ldx, (double *)p + (int64_t)c0 * ldp, ldp,
+                    (const double *)r + (int64_t)c0 * ldr, ldr, kxc, kpc, s);
+            else
+                k_cg_search<float><<<grid, kBlock, 0, ctx().stream>>>(
+                    n, (float *)x + (int64_t)c0 * ldx, ldx, (float *)p + (int64_t)c0 * ldp, ldp,
+                    (const float *)r + (int64_t)c0 * ldr, ldr, kxc, kpc, s);
+        }
+        if (!check(hipGetLastError(), "k_cg_search")) return;
+    }
 }
 
 bool sampled_outer(int prec, HIP_MAT *A, const void *X, int64_t ldx, int k, const double *w, double *g_dev)

@@ -119,7 +119,7 @@ __global__ __launch_bounds__(kThreads) void k_gram_mfma(int64_t n, const T *__re
     constexpr int S = 4 * R;  // rows per wavefront superstep
     constexpr int TP = GP * WP * 16, TQ = GQ * WQ * 16;
     using acc_t = typename MT<T>::acc_t;
-    __shared__ double red[GK > 1 ? (GK - 1) * WP * WQ * 4 * 64 : 1];
+    __shared__ double red[GK > 1 ? (GK - 1) * GP * GQ * WP * WQ * 4 * 64 : 1];
 
     const int ntile = tiles_p * tiles_q;
     const int logical = xcd_logical_block();
@@ -157,9 +157,6 @@ __global__ __launch_bounds__(kThreads) void k_gram_mfma(int64_t n, const T *__re
     const int64_t full_end = rb + ((re - rb) / S) * S;
     int64_t s = rb + (int64_t)wk * S;
     if (s < full_end) {
-        // supersteps in pairs with two operand buffers (loads of one in
-        // flight while the MFMAs of the other are issued)
-        vec_t a0[WP], b0[WQ], a1[WP], b1[WQ];
         auto load = [&](int64_t ss, vec_t (&aa)[WP], vec_t (&bb)[WQ]) {
 #pragma unroll
             for (int x = 0; x < WP; x++) aa[x] = *reinterpret_cast<const vec_t *>(vc[x] + ss + (int64_t)kk * R);
@@ -174,20 +171,33 @@ __global__ __launch_bounds__(kThreads) void k_gram_mfma(int64_t n, const T *__re
 #pragma unroll
                     for (int y = 0; y < WQ; y++) acc[x][y] = MT<T>::mma(aa[x][t], bb[y][t], acc[x][y]);
         };
-        // the last full superstep of this wavefront
-        const int64_t last_full = s + ((full_end - 1 - s) / step) * step;
-        load(s, a0, b0);
-        for (; s + step < full_end; s += 2 * step) {
-            load(s + step, a1, b1);
-            mma(a0, b0);
-            // unconditional (clamped to a full superstep) so the wait before
-            // the next MFMAs covers only the older loads
-            load(min(s + 2 * step, last_full), a0, b0);
-            mma(a1, b1);
-        }
-        if (s < full_end) {
-            mma(a0, b0);
-            s += step;
+        if constexpr (GP * GQ == 4) {
+            // 64 x 64 tiles: one operand buffer (register budget for four
+            // wavefronts per SIMD); latency is hidden by occupancy
+            for (; s < full_end; s += step) {
+                vec_t a0[WP], b0[WQ];
+                load(s, a0, b0);
+                mma(a0, b0);
+            }
+        } else {
+            // supersteps in pairs with two operand buffers (loads of one in
+            // flight while the MFMAs of the other are issued)
+            vec_t a0[WP], b0[WQ], a1[WP], b1[WQ];
+            // the last full superstep of this wavefront
+            const int64_t last_full = s + ((full_end - 1 - s) / step) * step;
+            load(s, a0, b0);
+            for (; s + step < full_end; s += 2 * step) {
+                load(s + step, a1, b1);
+                mma(a0, b0);
+                // unconditional (clamped to a full superstep) so the wait
+                // before the next MFMAs covers only the older loads
+                load(min(s + 2 * step, last_full), a0, b0);
+                mma(a1, b1);
+            }
+            if (s < full_end) {
+                mma(a0, b0);
+                s += step;
+            }
         }
     }
     // the partial superstep at the end of the chunk (rows past re read as 0)
@@ -206,11 +216,12 @@ __global__ __launch_bounds__(kThreads) void k_gram_mfma(int64_t n, const T *__re
                 for (int y = 0; y < WQ; y++) acc[x][y] = MT<T>::mma(a[x][t], b[y][t], acc[x][y]);
     }
 
-    // sum the GK row-split partial tiles in wavefront order:
-    // red[((wk - 1) * SLOTS + slot) * 64 + lane]
+    // sum the GK row-split partial tiles in wavefront order: the wavefronts
+    // of row slice wk > 0 park their tiles in red[((wk - 1) GP GQ + wpq) SLOTS
+    // + slot][lane], wpq = wp + GP wq, and slice 0 adds them
+    const int wpq = wp + GP * wq;
     if constexpr (GK > 1) {
         constexpr int SLOTS = WP * WQ * 4;
-        static_assert(GP * GQ == 1, "row-split Gram tiles use a single wavefront column");
         if (wk > 0) {
 #pragma unroll
             for (int x = 0; x < WP; x++)
@@ -218,7 +229,8 @@ __global__ __launch_bounds__(kThreads) void k_gram_mfma(int64_t n, const T *__re
                 for (int y = 0; y < WQ; y++)
 #pragma unroll
                     for (int t = 0; t < 4; t++)
-                        red[((size_t)(wk - 1) * SLOTS + (x * WQ + y) * 4 + t) * 64 + lane] = (double)acc[x][y][t];
+                        red[(((size_t)(wk - 1) * GP * GQ + wpq) * SLOTS + (x * WQ + y) * 4 + t) * 64 + lane] =
+                            (double)acc[x][y][t];
         }
         __syncthreads();
         if (wk > 0) return;
@@ -235,7 +247,8 @@ __global__ __launch_bounds__(kThreads) void k_gram_mfma(int64_t n, const T *__re
                 if constexpr (GK > 1) {
                     constexpr int SLOTS = WP * WQ * 4;
 #pragma unroll
-                    for (int w = 0; w < GK - 1; w++) v += red[((size_t)w * SLOTS + (x * WQ + y) * 4 + t) * 64 + lane];
+                    for (int w = 0; w < GK - 1; w++)
+                        v += red[(((size_t)w * GP * GQ + wpq) * SLOTS + (x * WQ + y) * 4 + t) * 64 + lane];
                 }
                 const int i = i0 + x * 16 + MT<T>::drow(kk, t), j = j0 + y * 16 + m;
                 if (i < p && j < q) out[(int64_t)j * p + i] = v;
@@ -314,6 +327,60 @@ __global__ __launch_bounds__(kThreads) void k_sum_chunks(const double *__restric
 // past p and q.  A wavefront owns rows [rb, rb + 32) as two 16-row blocks and
 // walks the output in groups of RN 16-column blocks; slot t of lane l holds
 // row drow(l / 16, t) of its block and column l % 16.
+// Result rows of one lane after regrouping: lane (g = l / 16, m) returns
+// rows 4 g .. 4 g + 3 of column m of the 16 x 16 block.  For FP32 these are
+// already its four slots; for FP64 (slot t holds row 4 t + g) the four lanes
+// of a column exchange values in four rounds: in round k, lane g reads the
+// slot g of lane (g + k) mod 4.
+template <typename T>
+__device__ __forceinline__ typename MT<T>::acc_t rows4(typename MT<T>::acc_t v, int g);
+
+template <>
+__device__ __forceinline__ fx4 rows4<float>(fx4 v, int)
+{
+    return v;
+}
+
+template <>
+__device__ __forceinline__ dx4 rows4<double>(dx4 v, int g)
+{
+    const int lane = threadIdx.x & 63, m = lane & 15;
+    dx4 w;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int src = (g + k) & 3;             // lane group this lane reads
+        const int give = (g - k) & 3;            // slot this lane sends: the reader's g
+        double send = v[0];
+        send = give == 1 ? v[1] : send;
+        send = give == 2 ? v[2] : send;
+        send = give == 3 ? v[3] : send;
+        const double got = __shfl(send, m + 16 * src, 64);
+        w[0] = src == 0 ? got : w[0];
+        w[1] = src == 1 ? got : w[1];
+        w[2] = src == 2 ? got : w[2];
+        w[3] = src == 3 ? got : w[3];
+    }
+    return w;
+}
+
+// y[0..3] = beta y + alpha v with the branch rules of y = a x + b y, as one
+// 16- (FP32) or 32-byte (FP64) access; y is 4-element aligned
+template <typename T>
+__device__ __forceinline__ void store_rows4(T *y, typename MT<T>::acc_t v, T al, T be)
+{
+    typedef T v4 __attribute__((ext_vector_type(4)));
+    v4 out;
+    if (be == T(0)) {
+#pragma unroll
+        for (int t = 0; t < 4; t++) out[t] = (al == T(1)) ? (T)v[t] : al * (T)v[t];
+    } else {
+        const v4 old = *reinterpret_cast<const v4 *>(y);
+#pragma unroll
+        for (int t = 0; t < 4; t++) out[t] = be * old[t] + al * (T)v[t];
+    }
+    *reinterpret_cast<v4 *>(y) = out;
+}
+
 template <typename T, int RN>
 __global__ __launch_bounds__(kThreads) void k_lincomb_mfma(int64_t n, const T *__restrict__ X, int64_t ldx, int p,
                                                            T *__restrict__ Y, int64_t ldy, int q,
@@ -387,20 +454,27 @@ __global__ __launch_bounds__(kThreads) void k_lincomb_mfma(int64_t n, const T *_
                 acc1[r] = MT<T>::mma(a1, bb, acc1[r]);
             }
         }
+        // epilogue: lane (kk, m) ends with rows rb + 16 h + 4 kk .. + 3 of
+        // column m in registers (the FP64 result rows are regrouped across
+        // the four lanes of a column first), written as one vector
 #pragma unroll
         for (int r = 0; r < RN; r++) {
             const int j = (jb0 + r) * 16 + m;
-            if (j >= q) continue;
-            T *yj = Y + (int64_t)j * ldy;
+            T *yj = Y + (int64_t)min(j, q - 1) * ldy;
 #pragma unroll
             for (int h = 0; h < 2; h++) {
+                const acc_t v = rows4<T>(h == 0 ? acc0[r] : acc1[r], kk);
+                const int64_t row = rb + 16 * h + 4 * kk;
+                if (j >= q) continue;
+                if (row + 4 <= n) {
+                    store_rows4<T>(yj + row, v, al, be);
+                } else {
 #pragma unroll
-                for (int t = 0; t < 4; t++) {
-                    const int64_t row = rb + 16 * h + MT<T>::drow(kk, t);
-                    if (row < n) {
-                        const T v = (T)(h == 0 ? acc0[r][t] : acc1[r][t]);
-                        yj[row] = (be == T(0)) ? ((al == T(1)) ? v : al * v) : be * yj[row] + al * v;
-                    }
+                    for (int t = 0; t < 4; t++)
+                        if (row + t < n) {
+                            const T vt = (T)v[t];
+                            yj[row + t] = (be == T(0)) ? ((al == T(1)) ? vt : al * vt) : be * yj[row + t] + al * vt;
+                        }
                 }
             }
         }
@@ -526,9 +600,11 @@ struct GramShape {
 
 GramShape gram_shape(int p, int q)
 {
-    if (p <= 16 && q <= 16) return {1, 1, 1, 1};
-    if (p <= 32 && q <= 32) return {2, 2, 1, 1};
-    return {2, 2, 2, 2};
+    if (p <= 16 && q <= 16) return {1, 1, 1, 1};  // 16 x 16 tiles, rows split 4 ways
+    if (p <= 32 && q <= 32) return {2, 2, 1, 1};  // 32 x 32, rows split 4 ways
+    if (q <= 16) return {2, 1, 2, 1};             // 64 x 16, rows split 2 ways
+    if (p <= 16) return {1, 2, 1, 2};             // 16 x 64, rows split 2 ways
+    return {2, 2, 2, 2};                          // 64 x 64
 }
 
 template <typename T>
@@ -543,6 +619,8 @@ void launch_gram_mfma(const GramShape &g, int grid, int64_t n, const void *V, in
     }
     GH_GRAM(1, 1, 1, 1)
     GH_GRAM(2, 2, 1, 1)
+    GH_GRAM(2, 1, 2, 1)
+    GH_GRAM(1, 2, 1, 2)
     GH_GRAM(2, 2, 2, 2)
 #undef GH_GRAM
 }
@@ -569,7 +647,7 @@ bool gram(int prec, int64_t n, const void *V, int64_t ldv, int p, const void *W,
         nchunk = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n + kThreads - 1) / kThreads));
     } else {
         if (!aligned32(V) || !aligned32(W) || (ldv * es) % 32 || (ldw * es) % 32) {
-            latch(160, "gram: operands are not 32-byte aligned");
+            latch(170, "gram: operands are not 32-byte aligned");
             return false;
         }
         g = gram_shape(p, q);
@@ -589,7 +667,8 @@ bool gram(int prec, int64_t n, const void *V, int64_t ldv, int p, const void *W,
     double *partial = (double *)scratch(sizeof(double) * ((size_t)nchunk * pq + pq));
     if (!partial) return false;
     {
-        ProfScope ps(K_GRAM, (double)n * (p + q) * es + 8.0 * pq, 2.0 * (double)n * pq);
+        ProfScope ps(thin ? K_GRAM_THIN : (prec == GCGE_HIP_FP64 ? K_GRAM_MFMA_F64 : K_GRAM_MFMA_F32),
+                     (double)n * (p + q) * es + 8.0 * pq, 2.0 * (double)n * pq);
         if (thin) {
             // the kernel wants p <= 4: swap the operands and transpose the output otherwise
             const bool swap = p > 4;
@@ -642,17 +721,21 @@ bool lincomb(int prec, int64_t n, const void *X, int64_t ldx, int p, void *Y, in
     if (p <= 8 && q <= 8) {
         if (((uintptr_t)X & 15u) || ((uintptr_t)Y & 15u) || (ldx * elem_size(prec)) % 16 ||
             (ldy * elem_size(prec)) % 16) {
-            latch(161, "lincomb: operands are not 16-byte aligned");
+            latch(171, "lincomb: operands are not 16-byte aligned");
             return false;
         }
         ThinCoef cf;
         for (int j = 0; j < q; j++)
             for (int i = 0; i < p; i++) cf.c[j * p + i] = C[(int64_t)j * ldc + i];
-        ProfScope ps(K_LINCOMB, (double)n * (p + q + (beta != 0.0 ? q : 0)) * elem_size(prec),
+        ProfScope ps(K_LINCOMB_THIN, (double)n * (p + q + (beta != 0.0 ? q : 0)) * elem_size(prec),
                      2.0 * (double)n * p * q);
         if (prec == GCGE_HIP_FP64) launch_lincomb_thin<double>(n, X, ldx, p, Y, ldy, q, cf, alpha, beta);
         else launch_lincomb_thin<float>(n, X, ldx, p, Y, ldy, q, cf, alpha, beta);
         return check(hipGetLastError(), "k_lincomb_thin");
+    }
+    if (((uintptr_t)Y & 31u) || (ldy * elem_size(prec)) % 32) {
+        latch(171, "lincomb: output is not 32-byte aligned");
+        return false;
     }
     const int nkb = (p + 3) / 4, nqb0 = (q + 15) / 16;
     const int rn = nqb0 <= 1 ? 1 : nqb0 <= 2 ? 2 : nqb0 <= 4 ? 4 : 8;
@@ -681,7 +764,7 @@ bool lincomb(int prec, int64_t n, const void *X, int64_t ldx, int p, void *Y, in
     if (!dC) return false;
     const int grid = (int)((n + 127) / 128);
     {
-        ProfScope ps(K_LINCOMB, (double)n * (p + q + (beta != 0.0 ? q : 0)) * elem_size(prec),
+        ProfScope ps(prec == GCGE_HIP_FP64 ? K_LINCOMB_MFMA_F64 : K_LINCOMB_MFMA_F32, (double)n * (p + q + (beta != 0.0 ? q : 0)) * elem_size(prec),
                      2.0 * (double)n * p * q);
 #define GH_LC(T_, RN_)                                                                                         \
     k_lincomb_mfma<T_, RN_><<<grid, kThreads, 0, ctx().stream>>>(n, (const T_ *)X, ldx, p, (T_ *)Y, ldy, q,    \

@@ -223,6 +223,21 @@ class Ops:
         _check("BlockCGUpdate")
         return rr
 
+    def cg_residual_update(self, W: MultiVec, R: MultiVec, k, alpha):
+        alpha = np.ascontiguousarray(alpha, dtype=np.float64)
+        rr = np.zeros(k)
+        self._f("BlockCGResidualUpdate", capi.F_BlockCGResidualUpdate)(W.ptr, R.ptr, k, _dp(alpha), _dp(rr),
+                                                                       self._p)
+        _check("BlockCGResidualUpdate")
+        return rr
+
+    def cg_search_update(self, X: MultiVec, xs, P: MultiVec, R: MultiVec, kx, kp, alpha, beta):
+        alpha = np.ascontiguousarray(alpha, dtype=np.float64)
+        beta = np.ascontiguousarray(beta, dtype=np.float64)
+        self._f("BlockCGSearchUpdate", capi.F_BlockCGSearchUpdate)(X.ptr, xs, P.ptr, R.ptr, kx, kp, _dp(alpha),
+                                                                   _dp(beta), self._p)
+        _check("BlockCGSearchUpdate")
+
     def residual_norms(self, A: Matrix, B, X: MultiVec, idx, lam):
         idx = np.ascontiguousarray(idx, dtype=np.int32)
         lam = np.ascontiguousarray(lam, dtype=np.float64)

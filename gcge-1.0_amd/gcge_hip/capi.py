@@ -95,7 +95,7 @@ OPS_SLOTS = [
     # block-kernel slots of this build
     "MultiVecAxpbyDiag", "MultiVecInnerProdDiag", "MultiVecLinearCombAxpby", "MatDotMultiVecInnerProdDiag",
     "BlockCGUpdate", "MultiVecResidualNorms", "MultiVecGather", "MultiVecCopy", "MultiVecNumRows",
-    "MultiVecCreateByMatPrecision",
+    "MultiVecCreateByMatPrecision", "BlockCGResidualUpdate", "BlockCGSearchUpdate",
 ]
 
 
@@ -199,6 +199,8 @@ F_MultiVecLinearCombAxpby = C.CFUNCTYPE(None, c_void_p, c_void_p, IP, IP, DP, c_
                                         c_void_p)
 F_MatDotMultiVecInnerProdDiag = C.CFUNCTYPE(None, c_void_p, c_void_p, c_void_p, IP, IP, DP, c_void_p)
 F_BlockCGUpdate = C.CFUNCTYPE(None, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int, DP, DP, c_void_p)
+F_BlockCGResidualUpdate = C.CFUNCTYPE(None, c_void_p, c_void_p, c_int, DP, DP, c_void_p)
+F_BlockCGSearchUpdate = C.CFUNCTYPE(None, c_void_p, c_int, c_void_p, c_void_p, c_int, c_int, DP, DP, c_void_p)
 F_MultiVecResidualNorms = C.CFUNCTYPE(None, c_void_p, c_void_p, c_void_p, IP, c_int, DP, DP, DP, DP, c_void_p)
 F_MultiVecGather = C.CFUNCTYPE(None, c_void_p, IP, c_int, c_void_p, c_int, c_void_p)
 F_MultiVecCopy = C.CFUNCTYPE(None, c_void_p, c_void_p, IP, IP, c_void_p)

@@ -135,6 +135,15 @@ typedef struct GCGE_OPS_ {
      * (0: FP64, 1: FP32); used for the FP32 inner solves (bcg_precision) */
     void (*MultiVecCreateByMatPrecision)(void ***multi_vec, GCGE_INT n_vec, void *mat, GCGE_INT precision,
                                          struct GCGE_OPS_ *ops);
+    /* the block CG step split for fusion (same roundings as BlockCGUpdate):
+     * r(:,j) -= alpha[j] w(:,j); rr[j] = <r(:,j), r(:,j)>, j < k */
+    void (*BlockCGResidualUpdate)(void **w, void **r, GCGE_INT k, const GCGE_DOUBLE *alpha, GCGE_DOUBLE *rr,
+                                  struct GCGE_OPS_ *ops);
+    /* x(:,sx+j) += alpha[j] p(:,j) for j < kx (the deferred solution update
+     * of the previous step), then p(:,j) = r(:,j) + beta[j] p(:,j) for
+     * j < kp, per element in that order (kx and kp in any relation) */
+    void (*BlockCGSearchUpdate)(void **x, GCGE_INT sx, void **p, void **r, GCGE_INT kx, GCGE_INT kp,
+                                const GCGE_DOUBLE *alpha, const GCGE_DOUBLE *beta, struct GCGE_OPS_ *ops);
 } GCGE_OPS;
 
 void GCGE_OPS_Create(GCGE_OPS **ops);

@@ -163,11 +163,13 @@ def test_gram_against_oracle(ops, p, q):
 
 
 # thin kernel (p <= 4, q <= 8 and the transpose), each MFMA workgroup shape
-# (16x16, 32x32, 64x64 tiles), partial tiles, and a row count that is not a
-# multiple of any superstep, on 100,489 rows so the rows split into chunks
+# (16x16, 32x32, 64x16, 16x64, 64x64 tiles), partial tiles, and a row count
+# that is not a multiple of any superstep, on 100,489 rows so the rows split
+# into chunks
 BIG = P.lap2d(317)[0]
 GRAM_SHAPES = [(1, 1), (1, 8), (8, 1), (4, 4), (3, 7), (7, 3), (6, 6), (12, 13), (16, 16), (17, 5),
-               (25, 25), (32, 32), (33, 40), (64, 64), (100, 100), (200, 100)]
+               (25, 25), (32, 32), (33, 40), (64, 64), (100, 100), (200, 100), (118, 12), (12, 118), (70, 16),
+               (9, 33)]
 
 
 @pytest.mark.parametrize("p,q", GRAM_SHAPES)
@@ -336,6 +338,42 @@ def test_cg_update_bit_exact(ops):
     np.testing.assert_array_equal(x.get(), xr)
     np.testing.assert_array_equal(r.get(), rr_ref)
     np.testing.assert_allclose(rr, np.einsum("ij,ij->j", rr_ref, rr_ref), rtol=RTOL_RED)
+
+
+def test_cg_split_steps_bit_exact(ops):
+    """BlockCGResidualUpdate and BlockCGSearchUpdate (the fused block CG
+    steps) against VecAxpby's roundings: r = -a w + r, x = a p + x, then
+    p = r + b p for the first kp columns only."""
+    A = MATS["vardiff2d"]
+    n, kx, kp = A.shape[0], 9, 6
+    g = rng(40)
+    X, P_, W, R = (g.standard_normal((n, kx)) for _ in range(4))
+    al = g.standard_normal(kx)
+    be = g.standard_normal(kx)
+    be[2] = 0.0
+    mA = Matrix(A)
+    x, p, w, r = (MultiVec(mA, kx, data=M) for M in (X, P_, W, R))
+    rr = ops.cg_residual_update(w, r, kx, al)
+    r_ref = (-al)[None, :] * W + R
+    np.testing.assert_array_equal(r.get(), r_ref)
+    np.testing.assert_allclose(rr, np.einsum("ij,ij->j", r_ref, r_ref), rtol=RTOL_RED)
+    ops.cg_search_update(x, 0, p, r, kx, kp, al, be)
+    x_ref = al[None, :] * P_ + X
+    p_ref = P_.copy()
+    p_ref[:, :kp] = r_ref[:, :kp] + be[None, :kp] * P_[:, :kp]
+    p_ref[:, 2] = r_ref[:, 2]          # beta == 0: p = r (copy branch)
+    np.testing.assert_array_equal(x.get(), x_ref)
+    np.testing.assert_array_equal(p.get(), p_ref)
+    # the active count may grow between steps (stale ratios of inactive
+    # columns): x for j < 3, p for j < 8
+    ops.cg_search_update(x, 0, p, r, 3, 8, al, be)
+    x2 = x_ref.copy()
+    x2[:, :3] = al[None, :3] * p_ref[:, :3] + x_ref[:, :3]
+    p2 = p_ref.copy()
+    p2[:, :8] = r_ref[:, :8] + be[None, :8] * p_ref[:, :8]
+    p2[:, 2] = r_ref[:, 2]
+    np.testing.assert_array_equal(x.get(), x2)
+    np.testing.assert_array_equal(p.get(), p2)
 
 
 @pytest.mark.parametrize("withB", [False, True])
