@@ -74,10 +74,11 @@ def kernel_delta(after, before):
 
 # profiler class -> the kernel template whose instantiations rocprofv3 lists
 ROCPROF_KERNEL = {
-    "spmm_f64": "k_spmm<double, false>", "spmm_f32": "k_spmm<float, false>",
-    "spmm_dot_f64": "k_spmm<double, true>", "spmm_dot_f32": "k_spmm<float, true>",
+    "spmm_f64": "k_spmm<double, false, ...>", "spmm_f32": "k_spmm<float, false, ...>",
+    "spmm_dot_f64": "k_spmm<double, true, ...>", "spmm_dot_f32": "k_spmm<float, true, ...>",
     "axpby": "k_axpby<...>", "dots": "k_dots<...>",
-    "cg_update_f64": "k_cg_update<double>", "cg_update_f32": "k_cg_update<float>",
+    "cg_update_f64": "k_cg_resid<double>", "cg_update_f32": "k_cg_resid<float>",
+    "cg_search_f64": "k_cg_search<double>", "cg_search_f32": "k_cg_search<float>",
     "gram_mfma_f64": "k_gram_mfma<double, ...>", "gram_mfma_f32": "k_gram_mfma<float, ...>",
     "gram_thin": "k_gram_thin<...>", "lincomb_mfma_f64": "k_lincomb_mfma<double, ...>",
     "lincomb_mfma_f32": "k_lincomb_mfma<float, ...>", "lincomb_thin": "k_lincomb_thin<...>",

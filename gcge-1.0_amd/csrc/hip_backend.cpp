@@ -161,7 +161,8 @@ const char *kclass_name(int k)
     // one class per kernel family and storage precision (k_spmm<T, DOT>,
     // k_cg_update<T>, k_gram_mfma<T, ...>, k_lincomb_mfma<T, ...>, ...)
     static const char *names[K_NUM] = {"spmm_f64", "spmm_f32", "spmm_dot_f64", "spmm_dot_f32", "axpby", "dots",
-                                       "cg_update_f64", "cg_update_f32", "gram_mfma_f64", "gram_mfma_f32",
+                                       "cg_update_f64", "cg_update_f32", "cg_search_f64", "cg_search_f32",
+                                       "gram_mfma_f64", "gram_mfma_f32",
                                        "gram_thin", "lincomb_mfma_f64", "lincomb_mfma_f32", "lincomb_thin",
                                        "residual", "copy", "swap", "other"};
     return (k >= 0 && k < K_NUM) ? names[k] : "?";

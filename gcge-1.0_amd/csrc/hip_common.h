@@ -96,6 +96,7 @@ bool sync();                           // stream synchronize + error check
 // launch; resolved at stream synchronisation points.
 enum KClass {
     K_SPMM_F64 = 0, K_SPMM_F32, K_SPMM_DOT_F64, K_SPMM_DOT_F32, K_AXPBY, K_DOTS, K_CG_UPDATE_F64, K_CG_UPDATE_F32,
+    K_CG_SEARCH_F64, K_CG_SEARCH_F32,
     K_GRAM_MFMA_F64, K_GRAM_MFMA_F32, K_GRAM_THIN, K_LINCOMB_MFMA_F64, K_LINCOMB_MFMA_F32, K_LINCOMB_THIN,
     K_RESIDUAL, K_COPY, K_SWAP, K_OTHER, K_NUM
 };

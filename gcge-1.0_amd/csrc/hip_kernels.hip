@@ -23,7 +23,6 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kSpmmRows = 256;   // rows per SpMM workgroup (one row per thread)
 constexpr int kSpmmCap = 2048;   // staged nonzeros per SpMM workgroup
-constexpr int kSpmmKC = 4;       // columns per SpMM register chunk
 
 struct ColScalars {
     double a[kMaxCols];
@@ -159,21 +158,32 @@ __global__ __launch_bounds__(kBlock) void k_zero_rows(int64_t n, T *__restrict__
 
 // ---------------------------------------------------------------- SpMM
 // One thread per row; the workgroup's nonzeros are staged in LDS when they
-// fit; columns are processed kSpmmKC at a time.  Each row sum starts at 0
-// and adds val*x in CSR order, as CSR_MatDotVec does.  With DOT, the
-// workgroup also writes partial[j * nblk + blk] = sum_r X(r,j) Y(r,j).
-template <typename T, bool DOT>
+// fit; columns are processed KC at a time.  Each row sum starts at 0 and
+// adds val*x in CSR order, as CSR_MatDotVec does.  With DOT, the workgroup
+// also writes partial[j * nblk + blk] = sum_r X(r,j) Y(r,j) (per-wavefront
+// sums combined in LDS in a fixed order).
+// STREAM (used for FP32): row blocks are renumbered so that each XCD walks a
+// contiguous range of rows (the plane neighbours of a stencil stay in its
+// L2), and the matrix and Y, each touched once, bypass the caches
+// (non-temporal) to leave L2 to X.
+template <typename T, bool DOT, int KC, bool STREAM>
 __global__ __launch_bounds__(kSpmmRows) void k_spmm(int n, const int *__restrict__ rowptr,
                                                     const int *__restrict__ col,
                                                     const T *__restrict__ val,
                                                     const T *__restrict__ X, int64_t ldx,
                                                     T *__restrict__ Y, int64_t ldy, int k,
-                                                    ColScalars g, double *__restrict__ partial)
+                                                    ColScalars g, double *__restrict__ partial, int nblk)
 {
     __shared__ int s_col[kSpmmCap];
     __shared__ T s_val[kSpmmCap];
-    __shared__ double sh[4];
-    const int r0 = blockIdx.x * kSpmmRows;
+    __shared__ double sdot[DOT ? kMaxCols : 1][4];
+    int blk = blockIdx.x;
+    if (STREAM) {
+        const int per = (int)(gridDim.x >> 3);
+        blk = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    }
+    if (blk >= nblk) return;  // uniform per workgroup (grid rounded up to 8)
+    const int r0 = blk * kSpmmRows;
     const int r = r0 + threadIdx.x;
     const int rend = min(r0 + kSpmmRows, n);
     const int e0 = rowptr[r0];
@@ -181,57 +191,70 @@ __global__ __launch_bounds__(kSpmmRows) void k_spmm(int n, const int *__restrict
     const bool staged = (e1 - e0) <= kSpmmCap;
     if (staged) {
         for (int e = e0 + threadIdx.x; e < e1; e += kSpmmRows) {
-            s_col[e - e0] = col[e];
-            s_val[e - e0] = val[e];
+            if (STREAM) {
+                s_col[e - e0] = __builtin_nontemporal_load(col + e);
+                s_val[e - e0] = __builtin_nontemporal_load(val + e);
+            } else {
+                s_col[e - e0] = col[e];
+                s_val[e - e0] = val[e];
+            }
         }
     }
     __syncthreads();
     const bool active = r < n;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     int s = 0, t = 0;
     if (active) {
         s = rowptr[r];
         t = rowptr[r + 1];
     }
-    for (int c0 = 0; c0 < k; c0 += kSpmmKC) {
-        const T *xb[kSpmmKC];
+    for (int c0 = 0; c0 < k; c0 += KC) {
+        const T *xb[KC];
 #pragma unroll
-        for (int kk = 0; kk < kSpmmKC; kk++) {
+        for (int kk = 0; kk < KC; kk++) {
             int c = min(c0 + kk, k - 1);
             int64_t xc = g.use_idx ? (int64_t)g.idx[c] : (int64_t)c;
             xb[kk] = X + xc * ldx;
         }
-        T acc[kSpmmKC];
+        T acc[KC];
 #pragma unroll
-        for (int kk = 0; kk < kSpmmKC; kk++) acc[kk] = T(0);
+        for (int kk = 0; kk < KC; kk++) acc[kk] = T(0);
         if (active) {
             if (staged) {
                 for (int e = s; e < t; e++) {
                     const int cj = s_col[e - e0];
                     const T v = s_val[e - e0];
 #pragma unroll
-                    for (int kk = 0; kk < kSpmmKC; kk++) acc[kk] = acc[kk] + v * xb[kk][cj];
+                    for (int kk = 0; kk < KC; kk++) acc[kk] = acc[kk] + v * xb[kk][cj];
                 }
             } else {
                 for (int e = s; e < t; e++) {
                     const int cj = col[e];
                     const T v = val[e];
 #pragma unroll
-                    for (int kk = 0; kk < kSpmmKC; kk++) acc[kk] = acc[kk] + v * xb[kk][cj];
+                    for (int kk = 0; kk < KC; kk++) acc[kk] = acc[kk] + v * xb[kk][cj];
                 }
             }
         }
 #pragma unroll
-        for (int kk = 0; kk < kSpmmKC; kk++) {
+        for (int kk = 0; kk < KC; kk++) {
             const int c = c0 + kk;
             if (c < k) {
-                if (active) Y[(int64_t)c * ldy + r] = acc[kk];
+                if (active) {
+                    if (STREAM) __builtin_nontemporal_store(acc[kk], Y + (int64_t)c * ldy + r);
+                    else Y[(int64_t)c * ldy + r] = acc[kk];
+                }
                 if (DOT) {
-                    double d = active ? (double)xb[kk][r] * (double)acc[kk] : 0.0;
-                    double bs = block_sum(d, sh);
-                    if (threadIdx.x == 0) partial[(int64_t)c * gridDim.x + blockIdx.x] = bs;
+                    const double d = wave_sum(active ? (double)xb[kk][r] * (double)acc[kk] : 0.0);
+                    if (lane == 0) sdot[c][wid] = d;
                 }
             }
         }
+    }
+    if (DOT) {
+        __syncthreads();
+        for (int c = threadIdx.x; c < k; c += kSpmmRows)
+            partial[(int64_t)c * nblk + blk] = (sdot[c][0] + sdot[c][1]) + (sdot[c][2] + sdot[c][3]);
     }
 }
 
@@ -552,8 +575,12 @@ static bool spmm_t(HIP_MAT *A, const void *X, int64_t ldx, void *Y, int64_t ldy,
             const bool f64 = sizeof(T) == 8;
             ProfScope ps(DOT ? (f64 ? K_SPMM_DOT_F64 : K_SPMM_DOT_F32) : (f64 ? K_SPMM_F64 : K_SPMM_F32), bytes,
                          2.0 * kc * (double)A->nnz);
-            k_spmm<T, DOT><<<nblk, kSpmmRows, 0, ctx().stream>>>(A->n, A->rowptr, A->col, val, Xc, ldx, Yc, ldy,
-                                                                  kc, g, partial);
+            if (f64)
+                k_spmm<T, DOT, 4, false><<<nblk, kSpmmRows, 0, ctx().stream>>>(A->n, A->rowptr, A->col, val, Xc, ldx,
+                                                                              Yc, ldy, kc, g, partial, nblk);
+            else
+                k_spmm<T, DOT, 8, true><<<((nblk + 7) / 8) * 8, kSpmmRows, 0, ctx().stream>>>(
+                    A->n, A->rowptr, A->col, val, Xc, ldx, Yc, ldy, kc, g, partial, nblk);
         }
         if (!check(hipGetLastError(), "k_spmm")) return false;
         if (DOT && !reduce_to_host(partial, kc, nblk, out + c0)) return false;
@@ -693,8 +720,8 @@ void launch_cg_search(int prec, int64_t n, void *x, int64_t ldx, void *p, int64_
         ColScalars s = make_scalars(kc, aa.data(), bb.data(), nullptr);
         dim3 grid(nb, kc);
         {
-            ProfScope ps(K_AXPBY, (3.0 * kxc + 3.0 * kpc - 1.0 * std::min(kxc, kpc)) * (double)n * elem_size(prec),
-                         0.0);
+            ProfScope ps(prec == GCGE_HIP_FP64 ? K_CG_SEARCH_F64 : K_CG_SEARCH_F32,
+                         (3.0 * kxc + 3.0 * kpc - 1.0 * std::min(kxc, kpc)) * (double)n * elem_size(prec), 0.0);
             if (prec == GCGE_HIP_FP64)
                 k_cg_search<double><<<grid, kBlock, 0, ctx().stream>>>(
                     n, (double *)x + (int64_t)c0 * ldx, ldx, (double *)p + (int64_t)c0 * ldp, ldp,

@@ -21,8 +21,9 @@
 // Very thin products (p <= 4, q <= 8 or the transpose) take a vector kernel
 // that keeps the whole p x q block in registers and reads V and W once.
 //
-// LinComb: a wavefront owns 32 rows of Y and walks the output columns in
-// blocks of 16; X supplies the A operand straight from HBM (16 consecutive
+// LinComb: a wavefront owns 32 rows of Y (with more than 64 output columns,
+// two wavefronts share the rows and split the columns) and walks
+// its output columns in blocks of 16; X supplies the A operand straight from HBM (16 consecutive
 // rows per lane group), C the B operand from a copy packed on the host in
 // operand order.  Thin combinations (p, q <= 8) take a vector kernel with C
 // in the kernel arguments and 16-byte row vectors.
@@ -381,7 +382,7 @@ __device__ __forceinline__ void store_rows4(T *y, typename MT<T>::acc_t v, T al,
     *reinterpret_cast<v4 *>(y) = out;
 }
 
-template <typename T, int RN>
+template <typename T, int RN, int WC>
 __global__ __launch_bounds__(kThreads) void k_lincomb_mfma(int64_t n, const T *__restrict__ X, int64_t ldx, int p,
                                                            T *__restrict__ Y, int64_t ldy, int q,
                                                            const T *__restrict__ Cp, int nkb, int nqb, double alpha,
@@ -390,7 +391,9 @@ __global__ __launch_bounds__(kThreads) void k_lincomb_mfma(int64_t n, const T *_
     using acc_t = typename MT<T>::acc_t;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int m = lane & 15, kk = lane >> 4;
-    const int64_t rb = ((int64_t)blockIdx.x * 4 + wave) * 32;
+    // WC wavefronts share one 32-row block and split its column groups
+    const int64_t rb = ((int64_t)blockIdx.x * (4 / WC) + wave / WC) * 32;
+    const int wcol = wave % WC;
     if (rb >= n) return;  // no workgroup barrier in this kernel
     // Rows past n are clamped to the last row: they feed only result rows
     // >= n, which are never stored.  nqb is a multiple of RN (Cp is padded
@@ -398,7 +401,7 @@ __global__ __launch_bounds__(kThreads) void k_lincomb_mfma(int64_t n, const T *_
     const int64_t ra0 = min(rb + m, n - 1), ra1 = min(rb + 16 + m, n - 1);
     const int nkb_full = p / 4;
     const T al = (T)alpha, be = (T)beta;
-    for (int jb0 = 0; jb0 < nqb; jb0 += RN) {
+    for (int jb0 = wcol * RN; jb0 < nqb; jb0 += WC * RN) {
         acc_t acc0[RN], acc1[RN];
 #pragma unroll
         for (int r = 0; r < RN; r++) {
@@ -737,9 +740,15 @@ bool lincomb(int prec, int64_t n, const void *X, int64_t ldx, int p, void *Y, in
         latch(171, "lincomb: output is not 32-byte aligned");
         return false;
     }
+    // column blocks per wavefront pass (RN) and wavefronts per row block
+    // (WC): more than 4 column blocks are split over two wavefronts (4 blocks
+    // each), which keeps the accumulators small enough for two or more
+    // wavefronts per SIMD
     const int nkb = (p + 3) / 4, nqb0 = (q + 15) / 16;
-    const int rn = nqb0 <= 1 ? 1 : nqb0 <= 2 ? 2 : nqb0 <= 4 ? 4 : 8;
-    const int nqb = ((nqb0 + rn - 1) / rn) * rn;  // zero blocks pad the last column group
+    const bool split = nqb0 > 4;
+    const int rn = split ? 4 : nqb0 <= 1 ? 1 : nqb0 <= 2 ? 2 : nqb0 <= 4 ? 4 : 8;
+    const int wc = split ? 2 : 1;
+    const int nqb = ((nqb0 + rn * wc - 1) / (rn * wc)) * rn * wc;  // zero blocks pad the last groups
     const size_t ncp = (size_t)nkb * nqb * 64;
     const void *dC = nullptr;
     if (prec == GCGE_HIP_FP64) {
@@ -762,25 +771,27 @@ bool lincomb(int prec, int64_t n, const void *X, int64_t ldx, int p, void *Y, in
         dC = stage_to_device(cp.data(), sizeof(float) * ncp);
     }
     if (!dC) return false;
-    const int grid = (int)((n + 127) / 128);
+    const int grid = (int)((n + 128 / wc - 1) / (128 / wc));
     {
         ProfScope ps(prec == GCGE_HIP_FP64 ? K_LINCOMB_MFMA_F64 : K_LINCOMB_MFMA_F32, (double)n * (p + q + (beta != 0.0 ? q : 0)) * elem_size(prec),
                      2.0 * (double)n * p * q);
-#define GH_LC(T_, RN_)                                                                                         \
-    k_lincomb_mfma<T_, RN_><<<grid, kThreads, 0, ctx().stream>>>(n, (const T_ *)X, ldx, p, (T_ *)Y, ldy, q,    \
-                                                                 (const T_ *)dC, nkb, nqb, alpha, beta)
+#define GH_LC(T_, RN_) GH_LCW(T_, RN_, 1)
+#define GH_LCW(T_, RN_, WC_)                                                                                   \
+    k_lincomb_mfma<T_, RN_, WC_><<<grid, kThreads, 0, ctx().stream>>>(n, (const T_ *)X, ldx, p, (T_ *)Y, ldy,  \
+                                                                      q, (const T_ *)dC, nkb, nqb, alpha, beta)
         if (prec == GCGE_HIP_FP64) {
-            if (rn == 1) GH_LC(double, 1);
+            if (wc == 2) GH_LCW(double, 4, 2);
+            else if (rn == 1) GH_LC(double, 1);
             else if (rn == 2) GH_LC(double, 2);
-            else if (rn == 4) GH_LC(double, 4);
-            else GH_LC(double, 8);
+            else GH_LC(double, 4);
         } else {
-            if (rn == 1) GH_LC(float, 1);
+            if (wc == 2) GH_LCW(float, 4, 2);
+            else if (rn == 1) GH_LC(float, 1);
             else if (rn == 2) GH_LC(float, 2);
-            else if (rn == 4) GH_LC(float, 4);
-            else GH_LC(float, 8);
+            else GH_LC(float, 4);
         }
 #undef GH_LC
+#undef GH_LCW
     }
     return check(hipGetLastError(), "k_lincomb");
 }
