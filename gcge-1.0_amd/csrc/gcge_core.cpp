@@ -1054,6 +1054,24 @@ extern "C" void GCGE_Solve(void *A, void *B, GCGE_DOUBLE *eval, void **evec, GCG
     int best_iter = para->num_iter;
     bool watching = false;
     para->stopped_on_stagnation = 0;
+    // lowest-max_res iterate while watched: eigenvectors (locked ones merged
+    // in from V) and eigenvalues
+    void **snap = nullptr;
+    std::vector<double> snap_eval, snap_pair_res;
+    double snap_res = 0.0;
+    auto take_snapshot = [&]() {
+        if (!snap) ops->MultiVecCreateByMat(&snap, nev, A, ops);
+        GCGE_INT s0[2] = {0, 0}, e0[2] = {nev, nev};
+        ops->MultiVecCopy(evec, snap, s0, e0, ops);
+        const int nsl = ws->num_soft_locked_in_last_iter;
+        if (nsl > 0) {
+            GCGE_INT e1[2] = {nsl, nsl};
+            ops->MultiVecCopy(V, snap, s0, e1, ops);
+        }
+        snap_eval.assign(ws->eval, ws->eval + nev);
+        snap_pair_res.assign(para->res, para->res + nev);
+        snap_res = para->max_res;
+    };
     while (para->num_unlock > 0 && para->num_iter < ev_max_it) {
         peak_res = std::max(peak_res, para->max_res);
         if (para->stagnation_window > 0 && para->stagnation_start > 0.0 &&
@@ -1063,6 +1081,11 @@ extern "C" void GCGE_Solve(void *A, void *B, GCGE_DOUBLE *eval, void **evec, GCG
             best_iter = para->num_iter;
         } else if (para->stagnation_window > 0) {
             watching = true;
+            if (!snap || para->max_res < snap_res) take_snapshot();
+            if (para->max_res > 10.0 * snap_res) {
+                para->stopped_on_stagnation = 2;
+                break;
+            }
             if (para->max_res < para->stagnation_ratio * best_res) {
                 best_res = para->max_res;
                 best_iter = para->num_iter;
@@ -1108,6 +1131,17 @@ extern "C" void GCGE_Solve(void *A, void *B, GCGE_DOUBLE *eval, void **evec, GCG
         if (para->print_result || para->print_eval) GCGE_PrintIterationInfo(ws->eval, para);
     }
 
+    if (snap && para->num_unlock > 0 && snap_res < para->max_res) {
+        // stopped without converging: hand back the best iterate seen
+        set2(ms, 0, 0);
+        set2(me, nev, nev);
+        ops->MultiVecCopy(snap, evec, ms, me, ops);
+        std::memcpy(ws->eval, snap_eval.data(), (size_t)nev * sizeof(double));
+        std::memcpy(para->res, snap_pair_res.data(), (size_t)nev * sizeof(double));
+        para->max_res = snap_res;
+        ws->num_soft_locked_in_last_iter = 0;  // the copy already holds the locked vectors
+    }
+    if (snap) ops->MultiVecDestroy(&snap, nev, ops);
     std::memcpy(eval, ws->eval, (size_t)nev * sizeof(double));
     set2(ms, 0, 0);
     set2(me, ws->num_soft_locked_in_last_iter, ws->num_soft_locked_in_last_iter);

@@ -119,10 +119,14 @@ typedef struct GCGE_PARA_ {
     GCGE_DOUBLE solve_seconds;   /* output: wall time of the last GCGE_Solve      */
     /* input: when > 0, the outer iteration also stops once max_res has not
      * fallen below stagnation_ratio times its best value for this many
-     * iterations (used by the FP32 phase of the mixed-precision solve) */
+     * iterations, or at once when it rises above 10 times its lowest value
+     * (a basis that has lost its orthonormality gives spurious Ritz
+     * values).  While watched, the iterate with the lowest max_res is kept
+     * in a copy and restored when the iteration stops without converging
+     * (used by the FP32 phase of the mixed-precision solve) */
     GCGE_INT stagnation_window;
     GCGE_DOUBLE stagnation_ratio;
-    GCGE_INT stopped_on_stagnation; /* output */
+    GCGE_INT stopped_on_stagnation; /* output: 1 stagnation, 2 breakdown */
     /* input: storage precision of the block-CG inner solves that build W
      * (0: same as the basis; 1: FP32 defect correction -- the initial
      * residual r0 = b - A W0 is formed in FP64, the CG for the correction

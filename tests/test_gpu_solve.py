@@ -12,6 +12,7 @@ a pair sits on a convergence threshold.
 import ast
 import glob
 import os
+import re
 
 import numpy as np
 import pytest
@@ -19,7 +20,7 @@ import scipy.sparse as sp
 
 import oracle_api as O
 from gcge_hip import problems as P
-from gcge_hip import solve
+from gcge_hip import Matrix, MultiVec, Ops, solve
 
 pytestmark = pytest.mark.gpu
 
@@ -168,6 +169,43 @@ def test_mixed_generalized_problem(precision):
     rmx = solve(A, B, nev, X0=X0, conv_type="NS", ev_tol=1e-9, ev_max_it=300, precision=precision)
     assert rmx["num_unlock"] == 0
     np.testing.assert_allclose(rmx["eval"], P.lap3d_mass_eigs(10, nev), rtol=EV_RTOL)
+
+
+def test_stagnation_stop_returns_best_iterate(capfd):
+    """An iteration watched for stagnation (stagnation_window > 0) that stops
+    without converging hands back its lowest-max_res iterate: the returned
+    residuals are those of the best watched iteration of the printed trace,
+    the residuals recomputed from the returned vectors and eigenvalues
+    reproduce them, and each eigenvalue is the Rayleigh quotient of its
+    vector.  The tolerance is out of reach, so the FP64 iteration ends at its
+    rounding floor, where max_res wanders from one iteration to the next; of
+    the runs below, at least one must stop on an iterate worse than its best,
+    so that the copy-back is exercised."""
+    A, _ = P.lap2d(48)
+    nev = 6
+    mA = Matrix(A)
+    restored = 0
+    for seed in (3, 4, 5):
+        for window in (3, 4, 6):
+            X0 = P.random_x0(A.shape[0], nev, seed)
+            r = solve(mA, None, nev, X0=X0, conv_type="NS", ev_tol=1e-30, ev_max_it=300,
+                      stagnation_window=window, stagnation_ratio=0.7, stagnation_start=1e-3, print_result=1)
+            trace = [float(v) for v in re.findall(r"max_res\(\s*\d+\) = (\S+);", capfd.readouterr().out)]
+            assert r["num_iter"] < 300, "the watch did not stop the iteration"
+            peak = np.maximum.accumulate(trace)
+            w = int(np.argmax(np.array(trace) <= 1e-3 * peak))
+            best = min(trace[w:])
+            restored += trace[-1] > best
+            np.testing.assert_allclose(r["res"].max(), best, rtol=1e-5)  # the trace prints 7 digits
+            X = r["evec"]
+            xv = MultiVec(mA, nev, data=X)
+            rn, xn, bn = Ops().residual_norms(mA, None, xv, np.arange(nev), r["eval"])
+            xv.close()
+            np.testing.assert_allclose(rn / (np.abs(r["eval"]) * bn), r["res"], rtol=1e-12)
+            rq = np.einsum("ij,ij->j", X, A @ X) / np.einsum("ij,ij->j", X, X)
+            np.testing.assert_allclose(r["eval"], rq, rtol=1e-10)
+    mA.close()
+    assert restored > 0, "every run stopped on its best iterate: the copy-back is not exercised"
 
 
 @pytest.mark.parametrize("withB", [False, True])
