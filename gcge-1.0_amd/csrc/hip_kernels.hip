@@ -157,15 +157,53 @@ __global__ __launch_bounds__(kBlock) void k_zero_rows(int64_t n, T *__restrict__
 }
 
 // ---------------------------------------------------------------- SpMM
+// Column sums of d[0..KC) over a wavefront by recursive halving: at level j
+// (partner lane ^ (32 >> j)) each lane keeps half of its columns (the upper
+// half if its bit is set) and adds the partner's copy of them; after log2 KC
+// levels it holds one column, lane >> (6 - log2 KC), which the remaining
+// levels sum over the lanes that share it.  7 exchanges for 4 columns, 10 for
+// 8, instead of 6 per column; the order is fixed, so the sums are
+// reproducible.
+template <int KC>
+__device__ __forceinline__ double wave_colsum(double (&d)[KC], int lane)
+{
+    static_assert(KC >= 1 && KC <= 64 && (KC & (KC - 1)) == 0, "KC is a power of two");
+    int m = KC;
+#pragma unroll
+    for (int mask = 32; mask >= 1; mask >>= 1) {
+        if (m > 1) {
+            const bool hi = (lane & mask) != 0;
+            const int h = m / 2;
+#pragma unroll
+            for (int i = 0; i < KC / 2; i++) {
+                if (i < h) {
+                    const double send = hi ? d[i] : d[i + h];
+                    const double keep = hi ? d[i + h] : d[i];
+                    d[i] = keep + __shfl_xor(send, mask, 64);
+                }
+            }
+            m = h;
+        } else {
+            d[0] = d[0] + __shfl_xor(d[0], mask, 64);
+        }
+    }
+    return d[0];
+}
+
 // One thread per row; the workgroup's nonzeros are staged in LDS when they
 // fit; columns are processed KC at a time.  Each row sum starts at 0 and
-// adds val*x in CSR order, as CSR_MatDotVec does.  With DOT, the workgroup
-// also writes partial[j * nblk + blk] = sum_r X(r,j) Y(r,j) (per-wavefront
-// sums combined in LDS in a fixed order).
+// adds val*x in CSR order, as CSR_MatDotVec does; the nonzeros go in batches
+// of kSpmmBatch whose column indices, values and X gathers are all issued
+// before the first product is added, so a row costs one memory round trip
+// per batch rather than one per nonzero.  With DOT, the workgroup also
+// writes partial[j * nblk + blk] = sum_r X(r,j) Y(r,j) (wavefront sums by
+// wave_colsum, combined in LDS in a fixed order).
 // STREAM (used for FP32): row blocks are renumbered so that each XCD walks a
 // contiguous range of rows (the plane neighbours of a stencil stay in its
 // L2), and the matrix and Y, each touched once, bypass the caches
 // (non-temporal) to leave L2 to X.
+constexpr int kSpmmBatch = 4;
+
 template <typename T, bool DOT, int KC, bool STREAM>
 __global__ __launch_bounds__(kSpmmRows) void k_spmm(int n, const int *__restrict__ rowptr,
                                                     const int *__restrict__ col,
@@ -208,6 +246,7 @@ __global__ __launch_bounds__(kSpmmRows) void k_spmm(int n, const int *__restrict
         s = rowptr[r];
         t = rowptr[r + 1];
     }
+    const int rs = active ? r : 0;  // a row index that is safe to gather
     for (int c0 = 0; c0 < k; c0 += KC) {
         const T *xb[KC];
 #pragma unroll
@@ -219,36 +258,55 @@ __global__ __launch_bounds__(kSpmmRows) void k_spmm(int n, const int *__restrict
         T acc[KC];
 #pragma unroll
         for (int kk = 0; kk < KC; kk++) acc[kk] = T(0);
-        if (active) {
-            if (staged) {
-                for (int e = s; e < t; e++) {
-                    const int cj = s_col[e - e0];
-                    const T v = s_val[e - e0];
+        for (int e = s; e < t; e += kSpmmBatch) {
+            const int cnt = t - e;  // > 0; the batch holds min(cnt, kSpmmBatch)
+            int cj[kSpmmBatch];
+            T v[kSpmmBatch];
 #pragma unroll
-                    for (int kk = 0; kk < KC; kk++) acc[kk] = acc[kk] + v * xb[kk][cj];
-                }
-            } else {
-                for (int e = s; e < t; e++) {
-                    const int cj = col[e];
-                    const T v = val[e];
-#pragma unroll
-                    for (int kk = 0; kk < KC; kk++) acc[kk] = acc[kk] + v * xb[kk][cj];
+            for (int u = 0; u < kSpmmBatch; u++) {
+                if (u < cnt) {
+                    if (staged) {
+                        cj[u] = s_col[e + u - e0];
+                        v[u] = s_val[e + u - e0];
+                    } else if (STREAM) {
+                        cj[u] = __builtin_nontemporal_load(col + e + u);
+                        v[u] = __builtin_nontemporal_load(val + e + u);
+                    } else {
+                        cj[u] = col[e + u];
+                        v[u] = val[e + u];
+                    }
+                } else {
+                    cj[u] = rs;
+                    v[u] = T(0);
                 }
             }
+            T xv[kSpmmBatch][KC];
+#pragma unroll
+            for (int u = 0; u < kSpmmBatch; u++)
+#pragma unroll
+                for (int kk = 0; kk < KC; kk++) xv[u][kk] = (u < cnt) ? xb[kk][cj[u]] : T(0);
+#pragma unroll
+            for (int u = 0; u < kSpmmBatch; u++)
+                if (u < cnt) {
+#pragma unroll
+                    for (int kk = 0; kk < KC; kk++) acc[kk] = acc[kk] + v[u] * xv[u][kk];
+                }
         }
+        double d[KC];
 #pragma unroll
         for (int kk = 0; kk < KC; kk++) {
             const int c = c0 + kk;
-            if (c < k) {
-                if (active) {
-                    if (STREAM) __builtin_nontemporal_store(acc[kk], Y + (int64_t)c * ldy + r);
-                    else Y[(int64_t)c * ldy + r] = acc[kk];
-                }
-                if (DOT) {
-                    const double d = wave_sum(active ? (double)xb[kk][r] * (double)acc[kk] : 0.0);
-                    if (lane == 0) sdot[c][wid] = d;
-                }
+            if (active && c < k) {
+                if (STREAM) __builtin_nontemporal_store(acc[kk], Y + (int64_t)c * ldy + r);
+                else Y[(int64_t)c * ldy + r] = acc[kk];
             }
+            if (DOT) d[kk] = (active && c < k) ? (double)xb[kk][r] * (double)acc[kk] : 0.0;
+        }
+        if (DOT) {
+            const double colsum = wave_colsum<KC>(d, lane);
+            constexpr int L = KC == 1 ? 0 : KC == 2 ? 1 : KC == 4 ? 2 : KC == 8 ? 3 : KC == 16 ? 4 : 5;
+            const int c = c0 + (lane >> (6 - L));
+            if ((lane & ((64 >> L) - 1)) == 0 && c < k) sdot[c][wid] = colsum;
         }
     }
     if (DOT) {
@@ -272,38 +330,92 @@ __global__ __launch_bounds__(kSpmmRows) void k_residual(int n, const int *__rest
                                                         const TX *__restrict__ X, int64_t ldx, int k,
                                                         ColScalars g, double *__restrict__ partial)
 {
-    __shared__ double sh[4];
-    const int r = blockIdx.x * kSpmmRows + threadIdx.x;
-    const bool active = r < n;
-    const int nb = gridDim.x;
-    for (int j = 0; j < k; j++) {
-        const TX *xj = X + (int64_t)g.idx[j] * ldx;
-        const double lam = g.a[j];
-        double rr = 0.0, xx = 0.0, bb = 0.0;
-        if (active) {
-            double ax = 0.0;
-            for (int e = arp[r]; e < arp[r + 1]; e++) ax = ax + aval[e] * (double)xj[acol[e]];
-            double bx;
-            if (HASB) {
-                bx = 0.0;
-                for (int e = brp[r]; e < brp[r + 1]; e++) bx = bx + bval[e] * (double)xj[bcol[e]];
-            } else {
-                bx = (double)xj[r];
-            }
-            const double res = axpby_el<double>(-lam, bx, 1.0, ax);
-            rr = res * res;
-            xx = (double)xj[r] * (double)xj[r];
-            bb = bx * bx;
-        }
-        double s1 = block_sum(rr, sh);
-        double s2 = block_sum(xx, sh);
-        double s3 = block_sum(bb, sh);
-        if (threadIdx.x == 0) {
-            partial[(int64_t)(3 * j + 0) * nb + blockIdx.x] = s1;
-            partial[(int64_t)(3 * j + 1) * nb + blockIdx.x] = s2;
-            partial[(int64_t)(3 * j + 2) * nb + blockIdx.x] = s3;
+    // The workgroup's rows of A are staged in LDS once and serve every
+    // column; columns go in chunks of kResKC, each thread one row, and the
+    // three sums of every column are reduced per wavefront into LDS and
+    // combined after the last chunk (same order as block_sum).
+    constexpr int kResKC = 4;
+    __shared__ int s_col[kSpmmCap];
+    __shared__ double s_val[kSpmmCap];
+    __shared__ double sred[3 * kMaxCols][4];
+    const int blk = blockIdx.x;
+    const int r0 = blk * kSpmmRows;
+    const int r = r0 + threadIdx.x;
+    const int rend = min(r0 + kSpmmRows, n);
+    const int e0 = arp[r0];
+    const int e1 = arp[rend];
+    const bool staged = (e1 - e0) <= kSpmmCap;
+    if (staged) {
+        for (int e = e0 + threadIdx.x; e < e1; e += kSpmmRows) {
+            s_col[e - e0] = acol[e];
+            s_val[e - e0] = aval[e];
         }
     }
+    __syncthreads();
+    const bool active = r < n;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int s = 0, t = 0;
+    if (active) {
+        s = arp[r];
+        t = arp[r + 1];
+    }
+    for (int c0 = 0; c0 < k; c0 += kResKC) {
+        const TX *xj[kResKC];
+#pragma unroll
+        for (int kk = 0; kk < kResKC; kk++) xj[kk] = X + (int64_t)g.idx[min(c0 + kk, k - 1)] * ldx;
+        double ax[kResKC];
+#pragma unroll
+        for (int kk = 0; kk < kResKC; kk++) ax[kk] = 0.0;
+        if (active) {
+            if (staged) {
+                for (int e = s; e < t; e++) {
+                    const int cj = s_col[e - e0];
+                    const double v = s_val[e - e0];
+#pragma unroll
+                    for (int kk = 0; kk < kResKC; kk++) ax[kk] = ax[kk] + v * (double)xj[kk][cj];
+                }
+            } else {
+                for (int e = s; e < t; e++) {
+                    const int cj = acol[e];
+                    const double v = aval[e];
+#pragma unroll
+                    for (int kk = 0; kk < kResKC; kk++) ax[kk] = ax[kk] + v * (double)xj[kk][cj];
+                }
+            }
+        }
+#pragma unroll
+        for (int kk = 0; kk < kResKC; kk++) {
+            const int c = c0 + kk;
+            if (c >= k) break;
+            double rr = 0.0, xx = 0.0, bb = 0.0;
+            if (active) {
+                const double xr = (double)xj[kk][r];
+                double bx;
+                if (HASB) {
+                    bx = 0.0;
+                    for (int e = brp[r]; e < brp[r + 1]; e++) bx = bx + bval[e] * (double)xj[kk][bcol[e]];
+                } else {
+                    bx = xr;
+                }
+                const double res = axpby_el<double>(-g.a[c], bx, 1.0, ax[kk]);
+                rr = res * res;
+                xx = xr * xr;
+                bb = bx * bx;
+            }
+            rr = wave_sum(rr);
+            xx = wave_sum(xx);
+            bb = wave_sum(bb);
+            if (lane == 0) {
+                sred[3 * c + 0][wid] = rr;
+                sred[3 * c + 1][wid] = xx;
+                sred[3 * c + 2][wid] = bb;
+            }
+        }
+    }
+    __syncthreads();
+    const int nb = gridDim.x;
+    for (int q = threadIdx.x; q < 3 * k; q += kSpmmRows)
+        partial[(int64_t)q * nb + blk] = (sred[q][0] + sred[q][1]) + (sred[q][2] + sred[q][3]);
 }
 
 // ---------------------------------------------------------------- CG update
@@ -578,6 +690,9 @@ static bool spmm_t(HIP_MAT *A, const void *X, int64_t ldx, void *Y, int64_t ldy,
             if (f64)
                 k_spmm<T, DOT, 4, false><<<nblk, kSpmmRows, 0, ctx().stream>>>(A->n, A->rowptr, A->col, val, Xc, ldx,
                                                                               Yc, ldy, kc, g, partial, nblk);
+            else if (DOT)
+                k_spmm<T, DOT, 4, true><<<((nblk + 7) / 8) * 8, kSpmmRows, 0, ctx().stream>>>(
+                    A->n, A->rowptr, A->col, val, Xc, ldx, Yc, ldy, kc, g, partial, nblk);
             else
                 k_spmm<T, DOT, 8, true><<<((nblk + 7) / 8) * 8, kSpmmRows, 0, ctx().stream>>>(
                     A->n, A->rowptr, A->col, val, Xc, ldx, Yc, ldy, kc, g, partial, nblk);

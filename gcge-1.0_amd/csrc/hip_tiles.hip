@@ -41,6 +41,7 @@ namespace {
 
 constexpr int kThreads = 256;       // 4 wavefronts per workgroup
 constexpr int kGramTargetWG = 2048; // workgroups a Gram launch aims for
+constexpr int kGramSlabWG = 512;    // workgroups of a slab Gram launch (two per CU)
 
 typedef double dx4 __attribute__((ext_vector_type(4)));
 typedef float fx4 __attribute__((ext_vector_type(4)));
@@ -254,6 +255,142 @@ __global__ __launch_bounds__(kThreads) void k_gram_mfma(int64_t n, const T *__re
                 const int i = i0 + x * 16 + MT<T>::drow(kk, t), j = j0 + y * 16 + m;
                 if (i < p && j < q) out[(int64_t)j * p + i] = v;
             }
+}
+
+// ---------------------------------------------------------------- Gram (LDS slabs)
+// Workgroup output block (MB 16) x (NB 16), NB <= 8 at run time, over one
+// row chunk.  Every SLAB rows (128 bytes of each column) the 256 threads copy
+// the block's columns of V and W into LDS (column-major, 16-byte pad per
+// column: 144-byte column stride, so neither the 16-byte copies nor the
+// operand reads of 16 consecutive columns meet a bank twice), two buffers
+// deep; wavefront w runs the MFMAs of tile columns w and w + 4 for all MB
+// tile rows.  V and W thus leave memory once per block instead of once per
+// 64 x 64 tile, and p, q are padded to 16 rather than 64.
+template <typename T, int MB>
+__global__ __launch_bounds__(kThreads) void k_gram_slab(int64_t n, const T *__restrict__ V, int64_t ldv, int p,
+                                                        const T *__restrict__ W, int64_t ldw, int q, int NB,
+                                                        int blocks_p, int blocks_q, int nchunk,
+                                                        int64_t rows_per_chunk, int upper_only,
+                                                        double *__restrict__ partial)
+{
+    constexpr int SLAB = 128 / (int)sizeof(T);     // rows per stage
+    constexpr int LD = SLAB + 16 / (int)sizeof(T);  // LDS column stride (elements)
+    constexpr int TP = MB * 16;
+    constexpr int COLS = TP + 128;                  // V block + up to 8 W tiles
+    constexpr int UPT = (COLS * 8 + kThreads - 1) / kThreads;  // 16-byte units per thread
+    constexpr int E = 16 / (int)sizeof(T);          // elements per unit
+    using acc_t = typename MT<T>::acc_t;
+    typedef T u16 __attribute__((ext_vector_type(E)));
+    __shared__ __attribute__((aligned(16))) T sbuf[2][COLS * LD];
+
+    const int nblock = blocks_p * blocks_q;
+    const int logical = xcd_logical_block();
+    if (logical >= nblock * nchunk) return;
+    const int chunk = logical / nblock, blk = logical % nblock;
+    const int i0 = (blk % blocks_p) * TP, j0 = (blk / blocks_p) * NB * 16;
+    const int64_t rb = (int64_t)chunk * rows_per_chunk;
+    const int64_t re = min(n, rb + rows_per_chunk);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int m = lane & 15, kk = lane >> 4;
+    const int ncols = TP + NB * 16;
+
+    // this thread's copy units: column c (V for c < TP, else W), 16 bytes
+    // at row offset E * seg of the slab; columns past p (q) repeat the last
+    // one and feed only result entries that are never stored
+    const T *src[UPT];
+    int dst[UPT];
+#pragma unroll
+    for (int u = 0; u < UPT; u++) {
+        const int unit = threadIdx.x + u * kThreads;
+        const int c = unit >> 3, seg = unit & 7;
+        if (c < ncols) {
+            src[u] = (c < TP ? V + (int64_t)min(i0 + c, p - 1) * ldv
+                             : W + (int64_t)min(j0 + c - TP, q - 1) * ldw) + seg * E;
+            dst[u] = c * LD + seg * E;
+        } else {
+            src[u] = nullptr;
+            dst[u] = -1;
+        }
+    }
+    u16 reg[UPT];
+    auto fetch = [&](int64_t r) {
+        if (r + SLAB <= re) {
+#pragma unroll
+            for (int u = 0; u < UPT; u++)
+                if (dst[u] >= 0) reg[u] = *reinterpret_cast<const u16 *>(src[u] + r);
+        } else {
+#pragma unroll
+            for (int u = 0; u < UPT; u++)
+                if (dst[u] >= 0) {
+                    const int64_t r0 = r + dst[u] % LD;  // row of the unit's first element
+#pragma unroll
+                    for (int e = 0; e < E; e++) reg[u][e] = (r0 + e < re) ? src[u][r + e] : T(0);
+                }
+        }
+    };
+    auto park = [&](int b) {
+#pragma unroll
+        for (int u = 0; u < UPT; u++)
+            if (dst[u] >= 0) *reinterpret_cast<u16 *>(&sbuf[b][dst[u]]) = reg[u];
+    };
+
+    acc_t acc[MB][2];
+#pragma unroll
+    for (int x = 0; x < MB; x++) acc[x][0] = acc[x][1] = acc_t{};
+    // tiles this wavefront computes: column y = wave + 4 yy; with upper_only,
+    // tiles entirely below the diagonal are skipped
+    bool live[MB][2];
+#pragma unroll
+    for (int yy = 0; yy < 2; yy++) {
+        const int y = wave + 4 * yy;
+#pragma unroll
+        for (int x = 0; x < MB; x++)
+            live[x][yy] = y < NB && !(upper_only && i0 + x * 16 > j0 + y * 16 + 15);
+    }
+
+    const int nslab = (int)((re - rb + SLAB - 1) / SLAB);
+    if (nslab > 0) {
+        fetch(rb);
+        park(0);
+    }
+    __syncthreads();
+    for (int sl = 0; sl < nslab; sl++) {
+        const int b = sl & 1;
+        const bool more = sl + 1 < nslab;
+        if (more) fetch(rb + (int64_t)(sl + 1) * SLAB);
+        const T *sv = sbuf[b];
+        const T *sw = sbuf[b] + TP * LD;
+#pragma unroll
+        for (int ks = 0; ks < SLAB / 4; ks++) {
+            const int kr = ks * 4 + kk;
+            T a[MB], bq[2];
+#pragma unroll
+            for (int x = 0; x < MB; x++) a[x] = sv[(x * 16 + m) * LD + kr];
+#pragma unroll
+            for (int yy = 0; yy < 2; yy++) bq[yy] = sw[((wave + 4 * yy) * 16 + m) * LD + kr];
+#pragma unroll
+            for (int yy = 0; yy < 2; yy++)
+#pragma unroll
+                for (int x = 0; x < MB; x++)
+                    if (live[x][yy]) acc[x][yy] = MT<T>::mma(a[x], bq[yy], acc[x][yy]);
+        }
+        if (more) park(b ^ 1);
+        __syncthreads();
+    }
+
+    double *out = partial + (int64_t)chunk * p * q;
+#pragma unroll
+    for (int yy = 0; yy < 2; yy++) {
+        const int y = wave + 4 * yy;
+        if (y >= NB) continue;
+#pragma unroll
+        for (int x = 0; x < MB; x++)
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int i = i0 + x * 16 + MT<T>::drow(kk, t), j = j0 + y * 16 + m;
+                if (i < p && j < q) out[(int64_t)j * p + i] = (double)acc[x][yy][t];
+            }
+    }
 }
 
 // ---------------------------------------------------------------- Gram (thin)
@@ -628,6 +765,20 @@ void launch_gram_mfma(const GramShape &g, int grid, int64_t n, const void *V, in
 #undef GH_GRAM
 }
 
+template <typename T>
+void launch_gram_slab(int MB, int grid, int64_t n, const void *V, int64_t ldv, int p, const void *W, int64_t ldw,
+                      int q, int NB, int bp, int bq, int nchunk, int64_t rpc, int upper, double *partial)
+{
+#define GH_SLAB(MB_)                                                                                            \
+    if (MB == MB_) {                                                                                            \
+        k_gram_slab<T, MB_><<<grid, kThreads, 0, ctx().stream>>>(n, (const T *)V, ldv, p, (const T *)W, ldw, q, \
+                                                                 NB, bp, bq, nchunk, rpc, upper, partial);      \
+        return;                                                                                                 \
+    }
+    GH_SLAB(1) GH_SLAB(2) GH_SLAB(3) GH_SLAB(4) GH_SLAB(5) GH_SLAB(6) GH_SLAB(7) GH_SLAB(8)
+#undef GH_SLAB
+}
+
 bool aligned32(const void *p) { return ((uintptr_t)p & 31u) == 0; }
 
 }  // namespace
@@ -646,8 +797,31 @@ bool gram(int prec, int64_t n, const void *V, int64_t ldv, int p, const void *W,
     int64_t rpc = 0;
     int tp = 1, tq = 1;
     GramShape g{1, 1, 1, 1};
+    // slab kernel beyond 64 columns on either side (blocks of up to 8 x 8
+    // tiles); up to 64 x 64 the single-tile workgroups with more chunks in
+    // flight move the data faster
+    const bool slab = !thin && (p > 64 || q > 64);
+    int MB = 0, NB = 0;
     if (thin) {
         nchunk = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n + kThreads - 1) / kThreads));
+    } else if (slab) {
+        if (!aligned32(V) || !aligned32(W) || (ldv * es) % 32 || (ldw * es) % 32) {
+            latch(170, "gram: operands are not 32-byte aligned");
+            return false;
+        }
+        const int p16 = (p + 15) / 16, q16 = (q + 15) / 16;
+        tp = (p16 + 7) / 8;
+        tq = (q16 + 7) / 8;
+        MB = (p16 + tp - 1) / tp;
+        NB = (q16 + tq - 1) / tq;
+        const int64_t slab_rows = 128 / (int64_t)es;
+        const int64_t max_chunks = std::max<int64_t>(1, (n + slab_rows - 1) / slab_rows);
+        // at most kGramSlabWG workgroups: all of them resident at once
+        nchunk = (int)std::min<int64_t>({(int64_t)kMaxChunks, max_chunks,
+                                         std::max<int64_t>(1, kGramSlabWG / (tp * tq))});
+        rpc = (n + nchunk - 1) / nchunk;
+        rpc = ((rpc + slab_rows - 1) / slab_rows) * slab_rows;
+        nchunk = (int)((n + rpc - 1) / rpc);
     } else {
         if (!aligned32(V) || !aligned32(W) || (ldv * es) % 32 || (ldw * es) % 32) {
             latch(170, "gram: operands are not 32-byte aligned");
@@ -682,6 +856,15 @@ bool gram(int prec, int64_t n, const void *V, int64_t ldv, int p, const void *W,
                 if (!swap) launch_thin_sized<float>(nchunk, n, V, ldv, p, W, ldw, q, 0, p, partial);
                 else launch_thin_sized<float>(nchunk, n, W, ldw, q, V, ldv, p, 1, p, partial);
             }
+        } else if (slab) {
+            const int64_t blocks = (int64_t)tp * tq * nchunk;
+            const int grid = (int)(((blocks + 7) / 8) * 8);
+            if (prec == GCGE_HIP_FP64)
+                launch_gram_slab<double>(MB, grid, n, V, ldv, p, W, ldw, q, NB, tp, tq, nchunk, rpc,
+                                         upper_only ? 1 : 0, partial);
+            else
+                launch_gram_slab<float>(MB, grid, n, V, ldv, p, W, ldw, q, NB, tp, tq, nchunk, rpc,
+                                        upper_only ? 1 : 0, partial);
         } else {
             const int64_t blocks = (int64_t)tp * tq * nchunk;
             const int grid = (int)(((blocks + 7) / 8) * 8);

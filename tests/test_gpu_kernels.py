@@ -162,14 +162,14 @@ def test_gram_against_oracle(ops, p, q):
     np.testing.assert_allclose(G, ref, rtol=RTOL_RED, atol=RTOL_RED * np.abs(ref).max())
 
 
-# thin kernel (p <= 4, q <= 8 and the transpose), each MFMA workgroup shape
-# (16x16, 32x32, 64x16, 16x64, 64x64 tiles), partial tiles, and a row count
-# that is not a multiple of any superstep, on 100,489 rows so the rows split
-# into chunks
+# thin kernel (p <= 4, q <= 8 and the transpose), the MFMA tile shapes up to
+# 32 x 32, the LDS-slab blocks beyond (one or several blocks per side, uneven
+# block splits), partial tiles, and a row count that is not a multiple of any
+# superstep or slab, on 100,489 rows so the rows split into chunks
 BIG = P.lap2d(317)[0]
 GRAM_SHAPES = [(1, 1), (1, 8), (8, 1), (4, 4), (3, 7), (7, 3), (6, 6), (12, 13), (16, 16), (17, 5),
                (25, 25), (32, 32), (33, 40), (64, 64), (100, 100), (200, 100), (118, 12), (12, 118), (70, 16),
-               (9, 33)]
+               (9, 33), (257, 129), (100, 260)]
 
 
 @pytest.mark.parametrize("p,q", GRAM_SHAPES)
@@ -185,7 +185,7 @@ def test_gram_tiles_large_n(ops, p, q):
     np.testing.assert_allclose(G, ref, rtol=RTOL_RED, atol=RTOL_RED * np.sqrt(n) * 10)
 
 
-@pytest.mark.parametrize("p,q", [(2, 5), (12, 13), (40, 70)])
+@pytest.mark.parametrize("p,q", [(2, 5), (12, 13), (40, 70), (130, 100)])
 def test_gram_fp32_tiles(ops, p, q):
     """FP32 data, FP32 accumulation inside a chunk, FP64 across chunks."""
     n = BIG.shape[0]
@@ -392,6 +392,33 @@ def test_residual_norms(ops, withB):
     np.testing.assert_allclose(rn, np.linalg.norm(R, axis=0), rtol=1e-11)
     np.testing.assert_allclose(xn, np.linalg.norm(X[:, idx], axis=0), rtol=1e-12)
     np.testing.assert_allclose(bn, np.linalg.norm(BX[:, idx], axis=0), rtol=1e-12)
+
+
+@pytest.mark.parametrize("case", ["vardiff3d_k7", "chung_lu_k130", "vardiff3d_f32_k9"])
+def test_residual_norms_column_chunks(ops, case):
+    """Ragged column counts (not a multiple of the kernel's column chunk, and
+    more than one launch's worth), workgroups whose rows hold more nonzeros
+    than the LDS stage (power-law graph: read straight from the CSR arrays)
+    and FP32 storage, against FP64 numpy."""
+    if case.startswith("chung_lu"):
+        A, _ = P.chung_lu(6000, seed=3)
+    else:
+        A, _ = P.vardiff3d(14, seed=2)
+    n = A.shape[0]
+    k = {"vardiff3d_k7": 7, "chung_lu_k130": 130, "vardiff3d_f32_k9": 9}[case]
+    X = rng(40).standard_normal((n, k + 3))
+    idx = rng(41).permutation(k + 3)[:k].astype(np.int32)
+    lam = rng(42).uniform(0.1, 10.0, k)
+    mA = Matrix(A)
+    f32 = case.endswith("f32_k9")
+    x = MultiVec(mA, k + 3, F32, data=X) if f32 else MultiVec(mA, k + 3, data=X)
+    if f32:
+        X = X.astype(np.float32).astype(np.float64)
+    rn, xn, bn = ops.residual_norms(mA, None, x, idx, lam)
+    R = A @ X[:, idx] - X[:, idx] * lam[None, :]
+    np.testing.assert_allclose(rn, np.linalg.norm(R, axis=0), rtol=1e-11)
+    np.testing.assert_allclose(xn, np.linalg.norm(X[:, idx], axis=0), rtol=1e-12)
+    np.testing.assert_allclose(bn, xn, rtol=0)
 
 
 def test_residual_norms_fp32_storage_fp64_math(ops):
