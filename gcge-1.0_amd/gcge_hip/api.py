@@ -267,8 +267,8 @@ def synchronize():
 # GCGE_HIP_MIXED_OPTS of the two-phase forms (0 selects the library's default for
 # fp32_tol, the iteration budgets -- the caller's ev_max_it -- and stagnation_ratio)
 MIXED_DEFAULTS = dict(fp32_tol=1e-7, fp32_max_it=0, fp64_max_it=0, stagnation_window=10, stagnation_ratio=0.7,
-                      fp32_w_orth=1, refine_w_orth=1, refine_bcg_precision=1, stagnation_start=1e-3)
-FP32_REFINE_DEFAULTS = dict(MIXED_DEFAULTS, fp32_w_orth=0, refine_w_orth=0, refine_bcg_precision=0)
+                      fp32_w_orth=1, refine_w_orth=1, refine_bcg_precision=1, stagnation_start=1e-3, cg_max_it=100)
+FP32_REFINE_DEFAULTS = dict(MIXED_DEFAULTS, fp32_w_orth=0, refine_w_orth=0, refine_bcg_precision=0, cg_max_it=0)
 
 
 def solve(A, B, nev, X0=None, precision="fp64", mixed=None, seed=None, keep_evec=True, **params):
@@ -281,7 +281,9 @@ def solve(A, B, nev, X0=None, precision="fp64", mixed=None, seed=None, keep_evec
       "mixed"        GCGE_HIP_SOLVER_SolveMixed: an FP32 phase (FP32 matrix
                      values and multivectors, CholQR W orthonormalization),
                      then FP64 refinement sweeps from its eigenvectors with
-                     FP32 inner solves;
+                     FP32 inner solves; inner solves of up to 100 CG steps
+                     in both phases (mixed=dict(cg_max_it=0) keeps the
+                     caller's cg_max_it);
       "fp32+refine"  the same two phases with the reference's own
                      orthonormalization and FP64 inner solves in the
                      refinement.

@@ -112,7 +112,8 @@ class GCGE_SOLVER(C.Structure):
 class GCGE_HIP_MIXED_OPTS(C.Structure):
     _fields_ = [("fp32_tol", c_double), ("fp32_max_it", c_int), ("fp64_max_it", c_int),
                 ("stagnation_window", c_int), ("stagnation_ratio", c_double), ("fp32_w_orth", c_int),
-                ("refine_w_orth", c_int), ("refine_bcg_precision", c_int), ("stagnation_start", c_double)]
+                ("refine_w_orth", c_int), ("refine_bcg_precision", c_int), ("stagnation_start", c_double),
+                ("cg_max_it", c_int)]
 
 
 class GCGE_HIP_MIXED_INFO(C.Structure):

@@ -87,6 +87,8 @@ typedef struct {
                                       of the FP64 residual (bcg_precision), 0: FP64           */
     GCGE_DOUBLE stagnation_start;  /* stagnation is watched once max_res has fallen to this
                                       fraction of its peak (0: from the first iteration)      */
+    GCGE_INT cg_max_it;       /* CG step cap of the block-CG inner solves in both phases
+                                 (0: the caller's para->cg_max_it)                            */
 } GCGE_HIP_MIXED_OPTS;
 
 typedef struct {
@@ -105,8 +107,11 @@ typedef struct {
  * applies to the refinement.  When given_init_evec is set, evec holds the
  * start block.  opts may be NULL: fp32_tol 1e-7, the caller's ev_max_it for
  * both phases, stagnation window 10 / ratio 0.7 / start 1e-3, CholQR in both
- * phases and FP32 inner solves in the refinement.  In opts, a zero fp32_tol,
- * budget or stagnation_ratio selects the same default. */
+ * phases, FP32 inner solves in the refinement and a CG step cap of 100 for
+ * the inner solves (inner steps are cheap in FP32, and a stronger inner
+ * solve shortens the outer iteration).  In opts, a zero fp32_tol, budget or
+ * stagnation_ratio selects the same default, and a zero cg_max_it keeps the
+ * caller's. */
 GCGE_INT GCGE_HIP_SOLVER_SolveMixed(GCGE_SOLVER *solver, const GCGE_HIP_MIXED_OPTS *opts,
                                     GCGE_HIP_MIXED_INFO *info);
 

@@ -245,7 +245,7 @@ def test_lincomb_against_oracle(ops, p, q):
 @pytest.mark.parametrize("p,q,alpha,beta", [(1, 17, 1.0, 0.0), (3, 40, -1.0, 1.0), (16, 16, 0.5, 0.0),
                                              (33, 130, -1.0, 1.0), (300, 100, 1.0, 0.0), (2, 3, 2.0, -0.5),
                                              (8, 8, -1.0, 1.0), (5, 7, 1.0, 0.0), (1, 1, 3.0, 0.0),
-                                             (9, 8, -1.0, 1.0)])
+                                             (9, 8, -1.0, 1.0), (65, 128, -1.0, 1.0)])
 def test_lincomb_tiles_large_n(ops, p, q, alpha, beta):
     n = BIG.shape[0]
     X = rng(100 + p).standard_normal((n, p))

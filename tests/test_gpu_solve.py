@@ -161,6 +161,23 @@ def test_mixed_precision_reaches_fp64_answer(precision):
     assert np.all(ns_residual(A, None, rmx["evec"], rmx["eval"]) <= 1e-8 * 1.01)
 
 
+def test_mixed_inner_cg_cap():
+    """GCGE_HIP_MIXED_OPTS.cg_max_it caps the block-CG inner solves of the
+    two-phase solve (the refinement's sweep count is reported), and the answer
+    is the FP64 one; the plain FP64 solve keeps the reference's cap of 20."""
+    A, _ = P.vardiff3d(16, seed=5)
+    nev = 8
+    X0 = P.random_x0(A.shape[0], nev, 9)
+    r64 = solve(A, None, nev, X0=X0, conv_type="NS", ev_tol=1e-8, ev_max_it=600)
+    rc = solve(A, None, nev, X0=X0, conv_type="NS", ev_tol=1e-8, ev_max_it=600, precision="mixed",
+               mixed=dict(cg_max_it=3, fp32_tol=1e-4))
+    assert rc["num_unlock"] == 0 and r64["num_unlock"] == 0
+    assert rc["bcg_iters_total"] <= 3 * rc["bcg_calls"]
+    assert r64["bcg_iters_total"] > 3 * r64["bcg_calls"]
+    np.testing.assert_allclose(rc["eval"], r64["eval"], rtol=EV_RTOL)
+    assert np.all(ns_residual(A, None, rc["evec"], rc["eval"]) <= 1e-8 * 1.01)
+
+
 @pytest.mark.parametrize("precision", MIXED_FORMS)
 def test_mixed_generalized_problem(precision):
     A, B = P.lap3d_mass(10)
