@@ -79,24 +79,27 @@ ROCPROF_KERNEL = {
     "axpby": "k_axpby<...>", "dots": "k_dots<...>",
     "cg_update_f64": "k_cg_resid<double>", "cg_update_f32": "k_cg_resid<float>",
     "cg_search_f64": "k_cg_search<double>", "cg_search_f32": "k_cg_search<float>",
-    "gram_mfma_f64": "k_gram_mfma<double, ...>", "gram_mfma_f32": "k_gram_mfma<float, ...>",
+    "gram_mfma_f64": "k_gram_mfma / k_gram_slab<double, ...>",
+    "gram_mfma_f32": "k_gram_mfma / k_gram_slab<float, ...>",
     "gram_thin": "k_gram_thin<...>", "lincomb_mfma_f64": "k_lincomb_mfma<double, ...>",
     "lincomb_mfma_f32": "k_lincomb_mfma<float, ...>", "lincomb_thin": "k_lincomb_thin<...>",
     "residual": "k_residual<...>", "copy": "k_copy<...>", "swap": "k_swap<...>",
 }
 
 
-def load_traffic():
-    """HBM bytes per launch measured with rocprofv3 --pmc (profiles/traffic.json)."""
+def load_traffic(run):
+    """HBM bytes per launch measured with rocprofv3 --pmc for the kernel
+    classes of one run ("mixed" or "fp64") of this workload
+    (profiles/traffic.json)."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
-            return json.load(f)
-    except (OSError, ValueError):
+            return json.load(f).get(run, {})
+    except (OSError, ValueError, AttributeError):
         return {}
 
 
-def roofline(stats):
+def roofline(stats, run):
     """Dominant kernel class (largest device time) and its achieved rate:
     algorithmic bytes (or flops, for the MFMA tiles) per launch / average
     launch time, from HIP events on the backend stream."""
@@ -107,7 +110,7 @@ def roofline(stats):
     if s["ms"] <= 0:
         return name, None
     cnt = max(1, s["count"])
-    tr = load_traffic().get(name)
+    tr = load_traffic(run).get(name)
     common = dict(kernel=ROCPROF_KERNEL.get(name, name), launches=s["count"],
                   avg_ms=round(s["ms"] / cnt, 5), bytes_per_launch=round(s["bytes"] / cnt),
                   traffic=(tr or {}).get("bytes_per_launch"))
@@ -195,10 +198,13 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--max-it", type=int, default=2000)
     ap.add_argument("--fp32-tol", type=float, default=1e-7)
-    ap.add_argument("--mode", default="mixed", choices=["mixed", "fp32+refine", "fp64-cg32"],
+    ap.add_argument("--mode", default="mixed", choices=["mixed", "fp32+refine", "fp64-cg32", "fp64"],
                     help="mixed: FP32 phase (CholQR) then FP64 refinement with FP32 inner solves; "
                          "fp32+refine: the same phases with the reference's orthonormalization and FP64 inner "
-                         "solves; fp64-cg32: FP64 with FP32 inner block-CG solves")
+                         "solves; fp64-cg32: FP64 with FP32 inner block-CG solves; fp64: the FP64 drop-in "
+                         "itself as the timed step (for profiling it; use with --no-fp64 --no-fp64-same)")
+    ap.add_argument("--verbose", action="store_true",
+                    help="print GCGE's per-iteration convergence line (stdout, before the JSON line)")
     ap.add_argument("--stagnation-window", type=int, default=10)
     ap.add_argument("--stagnation-start", type=float, default=1e-3)
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 solve (reference defaults)")
@@ -240,7 +246,7 @@ def main():
     L.GCGE_HIP_Synchronize()
     log(f"upload {time.perf_counter() - t:.2f}s")
 
-    common = dict(conv_type="NS", ev_tol=args.tol, ev_max_it=args.max_it)
+    common = dict(conv_type="NS", ev_tol=args.tol, ev_max_it=args.max_it, print_result=1 if args.verbose else 0)
     mixed = dict(fp32_tol=args.fp32_tol, fp32_max_it=args.max_it, fp64_max_it=args.max_it,
                  stagnation_window=args.stagnation_window, stagnation_ratio=0.7,
                  stagnation_start=args.stagnation_start, cg_max_it=args.cg_max_it)
@@ -306,7 +312,7 @@ def main():
         except Exception as e:  # the baseline must not hide the GPU result
             log(f"cpu baseline failed: {e}")
 
-    dom, roof = roofline(mixed_stats)
+    dom, roof = roofline(mixed_stats, "fp64" if args.mode == "fp64" else "mixed")
     line = {
         "metric": "Eigenpairs/sec to 1e-8 rel. residual, mixed vs FP64 GCGE, 1 MI355X",
         "value": round(value, 4),
@@ -318,7 +324,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64",
+        "dtype": "f64" if args.mode == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {"workload": f"{args.workload} size {args.size} (n={n}, nnz={nnz}), nev={nev}, B=I"
                                if B is None else f"{args.workload} size {args.size} (n={n}, nnz={nnz}+{B.nnz}), nev={nev}",
@@ -338,7 +344,7 @@ def main():
         "cpu_baseline": cpu,
     }
     if fp64:
-        d64, roof64 = roofline(fp64["stats"])
+        d64, roof64 = roofline(fp64["stats"], "fp64")
         line["fp64"] = dict(eigenpairs_per_s=round(nev / fp64["seconds"], 4), seconds=round(fp64["seconds"], 3),
                             iters=fp64["iters"], converged=fp64["unconverged"] == 0,
                             roofline_kernel=d64, roofline=roof64,
