@@ -11,10 +11,12 @@ A step is one complete mixed-precision solve from X0 with the matrix and X0
 already in HBM (default mode "mixed": an FP32 phase -- FP32 matrix values and
 multivectors, FP32 SpMM / block CG / MFMA Gram and LinComb tiles, CholQR W
 orthonormalization, FP64 dense Rayleigh-Ritz and residual checks -- then FP64
-refinement sweeps of GCG from the FP32 eigenvectors with FP32 inner solves).
-After the timed steps, FP64 solves from the same X0 are timed for the
-comparison: one with the same GCGE options as the mixed run, and one with the
-reference's defaults (the drop-in configuration).  The device kernels are
+refinement sweeps of GCG from the FP32 eigenvectors with FP32 inner solves;
+inner solves of up to --cg-max-it CG steps in both phases).  After the timed
+steps, FP64 solves from the same X0 are timed for the comparison: the drop-in
+with the reference's defaults ("fp64"), and FP64 with every algorithm setting
+of the mixed run -- CholQR, the same inner CG step cap -- so that only the
+precision differs ("fp64_same_settings").  The device kernels are
 timed with HIP events on the backend stream over the timed region; the CPU
 baseline is the reference built from its own sources (oracle/_ref/ref_solve)
 on one core, on a bounded sample (see ``cpu_baseline``).
@@ -231,7 +233,7 @@ def main():
         log("bench.py: only --gpus 1 is supported (one process drives one GPU)")
         sys.exit(2)
 
-    from gcge_hip import Matrix, capi, solve
+    from gcge_hip import Matrix, MultiVec, capi, solve
     from gcge_hip import problems as P
     L = capi.load()
 
@@ -243,8 +245,9 @@ def main():
     t = time.perf_counter()
     mA = Matrix(A)
     mB = Matrix(B) if B is not None else None
+    X0 = MultiVec(mA, nev, data=X0)  # resident in HBM before any timed solve
     L.GCGE_HIP_Synchronize()
-    log(f"upload {time.perf_counter() - t:.2f}s")
+    log(f"upload of the matrix and X0 over PCIe: {time.perf_counter() - t:.2f}s")
 
     common = dict(conv_type="NS", ev_tol=args.tol, ev_max_it=args.max_it, print_result=1 if args.verbose else 0)
     mixed = dict(fp32_tol=args.fp32_tol, fp32_max_it=args.max_it, fp64_max_it=args.max_it,
@@ -276,7 +279,7 @@ def main():
     converged = last["num_unlock"] == 0
 
     # residual check of the mixed result on the device, FP64 (NS criterion)
-    from gcge_hip import MultiVec, Ops
+    from gcge_hip import Ops
     ops = Ops()
     xv = MultiVec(mA, nev, data=last["evec"])
     rn, xn, bn = ops.residual_norms(mA, mB, xv, np.arange(nev), last["eval"])

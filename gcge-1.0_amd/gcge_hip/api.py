@@ -287,6 +287,7 @@ def solve(A, B, nev, X0=None, precision="fp64", mixed=None, seed=None, keep_evec
       "fp32+refine"  the same two phases with the reference's own
                      orthonormalization and FP64 inner solves in the
                      refinement.
+    X0 is an (n, nev) host array or an FP64 MultiVec already on the device.
     ``mixed`` overrides GCGE_HIP_MIXED_OPTS fields.  Returns a dict with
     eval, evec, res, num_iter, num_unlock, seconds (GCGE_Solve wall time, or
     the whole two-phase solve), bcg statistics and, for the two-phase forms,
@@ -322,7 +323,13 @@ def solve(A, B, nev, X0=None, precision="fp64", mixed=None, seed=None, keep_evec
     try:
         if sp.workspace.contents.V is None:
             raise RuntimeError("solver setup failed (missing operations)")
-        if X0 is not None:
+        if isinstance(X0, MultiVec):
+            # a block already in HBM: device-to-device copy into evec
+            s, e = _r((0, 0), (nev, nev))
+            fn = C.cast(sp.ops.contents.MultiVecCopy, capi.F_MultiVecCopy)
+            fn(X0.ptr, sp.evec, s, e, sp.ops)
+            _check("initial block")
+        elif X0 is not None:
             X0 = np.asfortranarray(X0, dtype=np.float64)
             L.HIP_MultiVecSetFromHost(sp.evec, 0, nev, _dp(X0), n)
             _check("initial block")

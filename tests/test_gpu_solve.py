@@ -161,6 +161,23 @@ def test_mixed_precision_reaches_fp64_answer(precision):
     assert np.all(ns_residual(A, None, rmx["evec"], rmx["eval"]) <= 1e-8 * 1.01)
 
 
+def test_device_initial_block_matches_host():
+    """X0 handed over as a device multivector (copied inside HBM) gives the
+    same solve as the host array: same iterations, same eigenvalues bit for
+    bit."""
+    A, _ = P.vardiff3d(12, seed=3)
+    nev = 6
+    X0 = P.random_x0(A.shape[0], nev, 4)
+    mA = Matrix(A)
+    rh = solve(mA, None, nev, X0=X0, conv_type="NS", ev_tol=1e-9, ev_max_it=300)
+    xd = MultiVec(mA, nev, data=X0)
+    rd = solve(mA, None, nev, X0=xd, conv_type="NS", ev_tol=1e-9, ev_max_it=300)
+    xd.close()
+    mA.close()
+    assert rd["num_iter"] == rh["num_iter"] and rd["num_unlock"] == 0
+    np.testing.assert_array_equal(rd["eval"], rh["eval"])
+
+
 def test_mixed_inner_cg_cap():
     """GCGE_HIP_MIXED_OPTS.cg_max_it caps the block-CG inner solves of the
     two-phase solve (the refinement's sweep count is reported), and the answer
