@@ -251,6 +251,19 @@ const float *mat_val32(HIP_MAT *A)
     return failed() ? nullptr : A->val32;
 }
 
+const void *mat_dinv(HIP_MAT *A, int prec)
+{
+    if (!A->dinv64) {
+        const size_t n = (size_t)std::max(1, A->n);
+        if (!check(hipMalloc((void **)&A->dinv64, sizeof(double) * n), "hipMalloc dinv64")) return nullptr;
+        if (!check(hipMalloc((void **)&A->dinv32, sizeof(float) * n), "hipMalloc dinv32")) return nullptr;
+        g_ctx.dev_bytes += 12.0 * (double)n;
+        launch_diag_inv(A);
+        if (failed()) return nullptr;
+    }
+    return prec == GCGE_HIP_FP64 ? (const void *)A->dinv64 : (const void *)A->dinv32;
+}
+
 const int *mat_rowidx(HIP_MAT *A)
 {
     if (A->rowidx) return A->rowidx;
@@ -453,6 +466,8 @@ extern "C" void HIP_MatDestroy(HIP_MAT **mat)
         if (A->val64) (void)hipFree(A->val64);
         if (A->val32) (void)hipFree(A->val32);
         if (A->rowidx) (void)hipFree(A->rowidx);
+        if (A->dinv64) (void)hipFree(A->dinv64);
+        if (A->dinv32) (void)hipFree(A->dinv32);
         g_ctx.dev_bytes -= sizeof(int) * (double)(A->n + 1) + 12.0 * A->nnz + (A->val32 ? 4.0 * A->nnz : 0.0) +
                            (A->rowidx ? 4.0 * A->nnz : 0.0);
         A->magic = 0;
@@ -960,6 +975,22 @@ void s_BlockCGResidualUpdate(void **w, void **r, GCGE_INT k, const GCGE_DOUBLE *
         for (int j = 0; j < k; j++) rr[j] = NAN;
 }
 
+void s_BlockPCGResidualUpdate(void *Mat, void **w, void **r, void **z, GCGE_INT k, const GCGE_DOUBLE *alpha,
+                              GCGE_DOUBLE *rz, GCGE_DOUBLE *rr, GCGE_OPS *)
+{
+    HIP_MAT *A = (HIP_MAT *)Mat;
+    MV *mw = as_mv(w), *mr = as_mv(r), *mz = as_mv(z);
+    const char *who = "BlockPCGResidualUpdate";
+    if (!range_ok(mw, 0, k, who) || !range_ok(mr, 0, k, who) || !range_ok(mz, 0, k, who) ||
+        !same_rows(mw, mr, who) || !same_rows(mr, mz, who) || !same_prec(mw, mr, who) || !same_prec(mr, mz, who) ||
+        !mat_ok(A, mr, who))
+        return;
+    const void *dinv = mat_dinv(A, mr->prec);
+    if (!dinv || !pcg_resid(mr->prec, mr->n, mw->data, mw->ld, mr->data, mr->ld, mz->data, mz->ld, dinv, k, alpha,
+                            rz, rr))
+        for (int j = 0; j < k; j++) rz[j] = rr[j] = NAN;
+}
+
 void s_BlockCGSearchUpdate(void **x, GCGE_INT sx, void **p, void **r, GCGE_INT kx, GCGE_INT kp,
                            const GCGE_DOUBLE *alpha, const GCGE_DOUBLE *beta, GCGE_OPS *)
 {
@@ -1091,6 +1122,7 @@ extern "C" void GCGE_HIP_SetOps(GCGE_OPS *ops)
     ops->BlockCGUpdate = s_BlockCGUpdate;
     ops->BlockCGResidualUpdate = s_BlockCGResidualUpdate;
     ops->BlockCGSearchUpdate = s_BlockCGSearchUpdate;
+    ops->BlockPCGResidualUpdate = s_BlockPCGResidualUpdate;
     ops->MultiVecResidualNorms = s_MultiVecResidualNorms;
     ops->MultiVecGather = s_MultiVecGather;
     ops->MultiVecCopy = s_MultiVecCopy;

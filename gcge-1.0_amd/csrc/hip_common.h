@@ -55,6 +55,8 @@ struct HIP_MAT_ {
     int *rowidx;      // device, nnz (row of each stored entry; built on demand)
     double *val64;    // device, nnz
     float *val32;     // device, nnz (built on demand)
+    double *dinv64;   // device, n: 1 / a_ii (1 where a_ii is 0 or absent; built on demand)
+    float *dinv32;    // device, n: the same, rounded to FP32
     int prec;         // precision of multivectors created from this matrix
     int max_row_nnz;
     double row_abs_max;  // max_i sum_j |a_ij| (Gershgorin bound of ||A||_2)
@@ -130,6 +132,8 @@ size_t elem_size(int prec);
 void *col_ptr(MV *m, int64_t c);
 const float *mat_val32(HIP_MAT *A);
 const int *mat_rowidx(HIP_MAT *A);
+// 1 / diagonal of A in the given precision (built on first use)
+const void *mat_dinv(HIP_MAT *A, int prec);
 
 // ---- launchers (hip_kernels.hip) --------------------------------------
 // All launchers validate nothing; callers check ranges first.
@@ -161,6 +165,11 @@ bool cg_update(int prec, int64_t n, void *x, int64_t ldx, const void *p, int64_t
 // r_j -= alpha_j w_j ; rr_j = <r_j, r_j>
 bool cg_resid(int prec, int64_t n, const void *w, int64_t ldw, void *r, int64_t ldr, int k, const double *alpha,
               double *rr);
+// r_j -= alpha_j w_j ; z_j = dinv .* r_j ; rz_j = <r_j, z_j> ; rr_j = <r_j, r_j>
+bool pcg_resid(int prec, int64_t n, const void *w, int64_t ldw, void *r, int64_t ldr, void *z, int64_t ldz,
+               const void *dinv, int k, const double *alpha, double *rz, double *rr);
+// dinv64[i] = 1 / a_ii (1 where a_ii is 0 or absent), dinv32 its FP32 rounding
+void launch_diag_inv(HIP_MAT *A);
 // x_j += alpha_j p_j (j < kx), then p_j = r_j + beta_j p_j (j < kp)
 void launch_cg_search(int prec, int64_t n, void *x, int64_t ldx, void *p, int64_t ldp, const void *r, int64_t ldr,
                       int kx, int kp, const double *alpha, const double *beta);

@@ -470,6 +470,53 @@ __global__ __launch_bounds__(kBlock) void k_cg_resid(int64_t n, const T *__restr
     if (threadIdx.x == 0) partial[(int64_t)j * gridDim.x + blockIdx.x] = bs;
 }
 
+// Jacobi-preconditioned residual step: r -= alpha w (same rounding as
+// k_cg_resid), z = dinv .* r, partial r.z (row 2j) and r.r (row 2j + 1)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pcg_resid(int64_t n, const T *__restrict__ w, int64_t ldw,
+                                                      T *__restrict__ r, int64_t ldr, T *__restrict__ z,
+                                                      int64_t ldz, const T *__restrict__ dinv, ColScalars s,
+                                                      double *__restrict__ partial)
+{
+    __shared__ double sh[4];
+    const int j = blockIdx.y;
+    const T al = (T)s.a[j];
+    const T *wj = w + (int64_t)j * ldw;
+    T *rj = r + (int64_t)j * ldr;
+    T *zj = z + (int64_t)j * ldz;
+    double arz = 0.0, arr = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+        const T rn = axpby_el<T>(-al, wj[i], T(1), rj[i]);
+        const T zn = rn * dinv[i];
+        rj[i] = rn;
+        zj[i] = zn;
+        arz = fma((double)rn, (double)zn, arz);
+        arr = fma((double)rn, (double)rn, arr);
+    }
+    const double brz = block_sum(arz, sh);
+    const double brr = block_sum(arr, sh);
+    if (threadIdx.x == 0) {
+        partial[(int64_t)(2 * j) * gridDim.x + blockIdx.x] = brz;
+        partial[(int64_t)(2 * j + 1) * gridDim.x + blockIdx.x] = brr;
+    }
+}
+
+// dinv[i] = 1 / a_ii; rows without a nonzero diagonal entry get 1 (no scaling)
+__global__ __launch_bounds__(kBlock) void k_diag_inv(int n, const int *__restrict__ rowptr,
+                                                     const int *__restrict__ col, const double *__restrict__ val,
+                                                     double *__restrict__ d64, float *__restrict__ d32)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    double d = 0.0;
+    for (int e = rowptr[i]; e < rowptr[i + 1]; e++)
+        if (col[e] == i) d = d + val[e];
+    const double inv = (d != 0.0) ? 1.0 / d : 1.0;
+    d64[i] = inv;
+    d32[i] = (float)inv;
+}
+
 // x += alpha p (VecAxpby(alpha, p, 1.0, x)) for the columns j < kx; then
 // p = r + beta p (MultiVecAxpbyDiag(1, r, beta, p)) for the columns j < kp
 template <typename T>
@@ -818,6 +865,48 @@ bool cg_resid(int prec, int64_t n, const void *w, int64_t ldw, void *r, int64_t 
         if (!reduce_to_host(partial, kc, nb, rr + c0)) return false;
     }
     return true;
+}
+
+bool pcg_resid(int prec, int64_t n, const void *w, int64_t ldw, void *r, int64_t ldr, void *z, int64_t ldz,
+               const void *dinv, int k, const double *alpha, double *rz, double *rr)
+{
+    if (k <= 0) return true;
+    if (failed()) return false;
+    const int nb = grid_1d(n, 256);
+    std::vector<double> tmp(2 * kMaxCols);
+    for (int c0 = 0; c0 < k; c0 += kMaxCols) {
+        const int kc = std::min(kMaxCols, k - c0);
+        ColScalars s = make_scalars(kc, alpha + c0, nullptr, nullptr);
+        double *partial = (double *)scratch(sizeof(double) * ((size_t)2 * kc * nb + 2 * kc));
+        if (!partial) return false;
+        dim3 grid(nb, kc);
+        {
+            ProfScope ps(prec == GCGE_HIP_FP64 ? K_CG_UPDATE_F64 : K_CG_UPDATE_F32,
+                         5.0 * kc * (double)n * elem_size(prec), 6.0 * kc * (double)n);
+            if (prec == GCGE_HIP_FP64)
+                k_pcg_resid<double><<<grid, kBlock, 0, ctx().stream>>>(
+                    n, (const double *)w + (int64_t)c0 * ldw, ldw, (double *)r + (int64_t)c0 * ldr, ldr,
+                    (double *)z + (int64_t)c0 * ldz, ldz, (const double *)dinv, s, partial);
+            else
+                k_pcg_resid<float><<<grid, kBlock, 0, ctx().stream>>>(
+                    n, (const float *)w + (int64_t)c0 * ldw, ldw, (float *)r + (int64_t)c0 * ldr, ldr,
+                    (float *)z + (int64_t)c0 * ldz, ldz, (const float *)dinv, s, partial);
+        }
+        if (!check(hipGetLastError(), "k_pcg_resid")) return false;
+        if (!reduce_to_host(partial, 2 * kc, nb, tmp.data())) return false;
+        for (int j = 0; j < kc; j++) {
+            rz[c0 + j] = tmp[2 * j];
+            rr[c0 + j] = tmp[2 * j + 1];
+        }
+    }
+    return true;
+}
+
+void launch_diag_inv(HIP_MAT *A)
+{
+    k_diag_inv<<<(A->n + kBlock - 1) / kBlock, kBlock, 0, ctx().stream>>>(A->n, A->rowptr, A->col, A->val64,
+                                                                         A->dinv64, A->dinv32);
+    check(hipGetLastError(), "k_diag_inv");
 }
 
 void launch_cg_search(int prec, int64_t n, void *x, int64_t ldx, void *p, int64_t ldp, const void *r, int64_t ldr,

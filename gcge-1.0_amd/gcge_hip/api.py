@@ -22,7 +22,7 @@ _CLI = {"ev_max_it", "block_size", "given_init_evec", "ev_tol", "conv_type", "co
         "max_reorth_time", "max_direct_orth_length", "multi_tol", "multi_tol_for_lock", "if_use_bcg",
         "cg_max_it", "cg_rate", "cg_type", "print_eval", "print_part_time", "print_para", "print_result",
         "opt_rr_eig_partly", "x_orth_block_size", "w_orth_block_size", "if_use_cg", "print_cg_error",
-        "print_orth_zero", "scbgs_wself_max_reorth_time", "bcg_precision", "lock_converged"}
+        "print_orth_zero", "scbgs_wself_max_reorth_time", "bcg_precision", "lock_converged", "bcg_precond"}
 # members set directly after initialisation (read at solve time)
 _DIRECT = {"dirichlet_boundary", "stagnation_window", "stagnation_ratio", "stagnation_start"}
 
@@ -231,6 +231,14 @@ class Ops:
         _check("BlockCGResidualUpdate")
         return rr
 
+    def pcg_residual_update(self, A: Matrix, W: MultiVec, R: MultiVec, Z: MultiVec, k, alpha):
+        alpha = np.ascontiguousarray(alpha, dtype=np.float64)
+        rz, rr = np.zeros(k), np.zeros(k)
+        self._f("BlockPCGResidualUpdate", capi.F_BlockPCGResidualUpdate)(A.ptr, W.ptr, R.ptr, Z.ptr, k, _dp(alpha),
+                                                                         _dp(rz), _dp(rr), self._p)
+        _check("BlockPCGResidualUpdate")
+        return rz, rr
+
     def cg_search_update(self, X: MultiVec, xs, P: MultiVec, R: MultiVec, kx, kp, alpha, beta):
         alpha = np.ascontiguousarray(alpha, dtype=np.float64)
         beta = np.ascontiguousarray(beta, dtype=np.float64)
@@ -267,8 +275,10 @@ def synchronize():
 # GCGE_HIP_MIXED_OPTS of the two-phase forms (0 selects the library's default for
 # fp32_tol, the iteration budgets -- the caller's ev_max_it -- and stagnation_ratio)
 MIXED_DEFAULTS = dict(fp32_tol=1e-7, fp32_max_it=0, fp64_max_it=0, stagnation_window=10, stagnation_ratio=0.7,
-                      fp32_w_orth=1, refine_w_orth=1, refine_bcg_precision=1, stagnation_start=1e-3, cg_max_it=100)
-FP32_REFINE_DEFAULTS = dict(MIXED_DEFAULTS, fp32_w_orth=0, refine_w_orth=0, refine_bcg_precision=0, cg_max_it=0)
+                      fp32_w_orth=1, refine_w_orth=1, refine_bcg_precision=1, stagnation_start=1e-3, cg_max_it=30,
+                      bcg_precond=1)
+FP32_REFINE_DEFAULTS = dict(MIXED_DEFAULTS, fp32_w_orth=0, refine_w_orth=0, refine_bcg_precision=0, cg_max_it=0,
+                            bcg_precond=0)
 
 
 def solve(A, B, nev, X0=None, precision="fp64", mixed=None, seed=None, keep_evec=True, **params):
@@ -281,9 +291,10 @@ def solve(A, B, nev, X0=None, precision="fp64", mixed=None, seed=None, keep_evec
       "mixed"        GCGE_HIP_SOLVER_SolveMixed: an FP32 phase (FP32 matrix
                      values and multivectors, CholQR W orthonormalization),
                      then FP64 refinement sweeps from its eigenvectors with
-                     FP32 inner solves; inner solves of up to 100 CG steps
-                     in both phases (mixed=dict(cg_max_it=0) keeps the
-                     caller's cg_max_it);
+                     FP32 inner solves; inner solves preconditioned with the
+                     diagonal of A and capped at 30 CG steps in both phases
+                     (mixed=dict(cg_max_it=0, bcg_precond=0) keeps the
+                     caller's settings);
       "fp32+refine"  the same two phases with the reference's own
                      orthonormalization and FP64 inner solves in the
                      refinement.

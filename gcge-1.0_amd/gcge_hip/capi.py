@@ -65,7 +65,7 @@ class GCGE_PARA(C.Structure):
         ("solve_seconds", c_double),
         ("stagnation_window", c_int), ("stagnation_ratio", c_double), ("stopped_on_stagnation", c_int),
         ("bcg_precision", c_int), ("stagnation_start", c_double), ("lock_converged", c_int),
-        ("basis_eps", c_double),
+        ("basis_eps", c_double), ("bcg_precond", c_int),
     ]
 
 
@@ -80,6 +80,7 @@ class GCGE_WORKSPACE(C.Structure):
         ("num_soft_locked_in_last_iter", c_int), ("dense_matrix", c_void_p),
         ("subspace_matrix", DP), ("subspace_evec", DP), ("subspace_dtmp", DP), ("subspace_itmp", IP),
         ("eval", DP), ("orth_ind", IP), ("bcg32", c_void_p * 4), ("bcg32_cols", c_int),
+        ("pcg_z", c_void_p), ("pcg_z32", c_void_p), ("pcg_z_cols", c_int), ("pcg_z32_cols", c_int),
     ]
 
 
@@ -95,7 +96,7 @@ OPS_SLOTS = [
     # block-kernel slots of this build
     "MultiVecAxpbyDiag", "MultiVecInnerProdDiag", "MultiVecLinearCombAxpby", "MatDotMultiVecInnerProdDiag",
     "BlockCGUpdate", "MultiVecResidualNorms", "MultiVecGather", "MultiVecCopy", "MultiVecNumRows",
-    "MultiVecCreateByMatPrecision", "BlockCGResidualUpdate", "BlockCGSearchUpdate",
+    "MultiVecCreateByMatPrecision", "BlockCGResidualUpdate", "BlockCGSearchUpdate", "BlockPCGResidualUpdate",
 ]
 
 
@@ -113,7 +114,7 @@ class GCGE_HIP_MIXED_OPTS(C.Structure):
     _fields_ = [("fp32_tol", c_double), ("fp32_max_it", c_int), ("fp64_max_it", c_int),
                 ("stagnation_window", c_int), ("stagnation_ratio", c_double), ("fp32_w_orth", c_int),
                 ("refine_w_orth", c_int), ("refine_bcg_precision", c_int), ("stagnation_start", c_double),
-                ("cg_max_it", c_int)]
+                ("cg_max_it", c_int), ("bcg_precond", c_int)]
 
 
 class GCGE_HIP_MIXED_INFO(C.Structure):
@@ -202,6 +203,7 @@ F_MatDotMultiVecInnerProdDiag = C.CFUNCTYPE(None, c_void_p, c_void_p, c_void_p, 
 F_BlockCGUpdate = C.CFUNCTYPE(None, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int, DP, DP, c_void_p)
 F_BlockCGResidualUpdate = C.CFUNCTYPE(None, c_void_p, c_void_p, c_int, DP, DP, c_void_p)
 F_BlockCGSearchUpdate = C.CFUNCTYPE(None, c_void_p, c_int, c_void_p, c_void_p, c_int, c_int, DP, DP, c_void_p)
+F_BlockPCGResidualUpdate = C.CFUNCTYPE(None, c_void_p, c_void_p, c_void_p, c_void_p, c_int, DP, DP, DP, c_void_p)
 F_MultiVecResidualNorms = C.CFUNCTYPE(None, c_void_p, c_void_p, c_void_p, IP, c_int, DP, DP, DP, DP, c_void_p)
 F_MultiVecGather = C.CFUNCTYPE(None, c_void_p, IP, c_int, c_void_p, c_int, c_void_p)
 F_MultiVecCopy = C.CFUNCTYPE(None, c_void_p, c_void_p, IP, IP, c_void_p)

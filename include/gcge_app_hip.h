@@ -89,6 +89,8 @@ typedef struct {
                                       fraction of its peak (0: from the first iteration)      */
     GCGE_INT cg_max_it;       /* CG step cap of the block-CG inner solves in both phases
                                  (0: the caller's para->cg_max_it)                            */
+    GCGE_INT bcg_precond;     /* 1: precondition the inner solves of both phases with the
+                                 diagonal of A (Jacobi); 0: the caller's para->bcg_precond  */
 } GCGE_HIP_MIXED_OPTS;
 
 typedef struct {
@@ -107,11 +109,10 @@ typedef struct {
  * applies to the refinement.  When given_init_evec is set, evec holds the
  * start block.  opts may be NULL: fp32_tol 1e-7, the caller's ev_max_it for
  * both phases, stagnation window 10 / ratio 0.7 / start 1e-3, CholQR in both
- * phases, FP32 inner solves in the refinement and a CG step cap of 100 for
- * the inner solves (inner steps are cheap in FP32, and a stronger inner
- * solve shortens the outer iteration).  In opts, a zero fp32_tol, budget or
- * stagnation_ratio selects the same default, and a zero cg_max_it keeps the
- * caller's. */
+ * phases, FP32 inner solves in the refinement, and inner solves
+ * preconditioned with the diagonal of A and capped at 30 CG steps.  In opts,
+ * a zero fp32_tol, budget or stagnation_ratio selects the same default, and a
+ * zero cg_max_it or bcg_precond keeps the caller's. */
 GCGE_INT GCGE_HIP_SOLVER_SolveMixed(GCGE_SOLVER *solver, const GCGE_HIP_MIXED_OPTS *opts,
                                     GCGE_HIP_MIXED_INFO *info);
 

@@ -146,6 +146,11 @@ typedef struct GCGE_PARA_ {
      * FLT_EPSILON for FP32); sets the drop and check tolerances of the
      * "cholqr" W orthonormalization */
     GCGE_DOUBLE basis_eps;
+    /* input: 1 preconditions the block CG that builds W with the diagonal of
+     * A (Jacobi); 0, the default, is the reference's unpreconditioned BCG.
+     * The stopping rule stays ||r_k|| / ||r_0|| < cg_rate.  Needs the
+     * BlockPCGResidualUpdate slot. */
+    GCGE_INT bcg_precond;
 } GCGE_PARA;
 
 void GCGE_PARA_Create(GCGE_PARA **para);

@@ -376,6 +376,45 @@ def test_cg_split_steps_bit_exact(ops):
     np.testing.assert_array_equal(p.get(), p2)
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_pcg_residual_update(ops, prec):
+    """BlockPCGResidualUpdate: r as BlockCGResidualUpdate's (bit for bit),
+    z = r * (1 / a_ii) with 1 / a_ii rounded once in FP64 (then to FP32 for
+    FP32 data), and r.z, r.r accumulated in FP64; a zero alpha leaves r as
+    it is.  The matrix has a row without a stored diagonal entry (z = r
+    there)."""
+    A = MATS["vardiff2d"].tolil()
+    A[5, 5] = 0.0
+    A = A.tocsr()
+    A.eliminate_zeros()
+    n, k = A.shape[0], 7
+    g = rng(44)
+    W, R = g.standard_normal((n, k)), g.standard_normal((n, k))
+    al = g.standard_normal(k)
+    f32 = prec == "f32"
+    pr = F32 if f32 else capi.GCGE_HIP_FP64
+    mA = Matrix(A)
+    w, r, z = MultiVec(mA, k, pr, data=W), MultiVec(mA, k, pr, data=R), MultiVec(mA, k, pr)
+    d = A.diagonal()
+    dinv = np.where(d != 0.0, 1.0 / np.where(d != 0.0, d, 1.0), 1.0)
+    if f32:
+        W, R = W.astype(np.float32), R.astype(np.float32)
+        r_ref = (-al.astype(np.float32))[None, :] * W + R
+        z_ref = r_ref * dinv.astype(np.float32)[:, None]
+    else:
+        r_ref = (-al)[None, :] * W + R
+        z_ref = r_ref * dinv[:, None]
+    rz, rr = ops.pcg_residual_update(mA, w, r, z, k, al)
+    np.testing.assert_array_equal(r.get(), r_ref.astype(np.float64))
+    np.testing.assert_array_equal(z.get(), z_ref.astype(np.float64))
+    rd, zd = r_ref.astype(np.float64), z_ref.astype(np.float64)
+    np.testing.assert_allclose(rz, np.einsum("ij,ij->j", rd, zd), rtol=RTOL_RED)
+    np.testing.assert_allclose(rr, np.einsum("ij,ij->j", rd, rd), rtol=RTOL_RED)
+    rz0, rr0 = ops.pcg_residual_update(mA, w, r, z, k, np.zeros(k))
+    np.testing.assert_array_equal(r.get(), rd)
+    np.testing.assert_allclose(rr0, rr, rtol=0)
+
+
 @pytest.mark.parametrize("withB", [False, True])
 def test_residual_norms(ops, withB):
     A, B = P.lap3d_mass(9)
