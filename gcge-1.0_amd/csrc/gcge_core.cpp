@@ -550,14 +550,15 @@ void cg_single(void *A, void *b, void *x, GCGE_OPS *ops, GCGE_PARA *para, void *
 // unconverged ones.  That rule is kept.
 // x_is_zero: the initial guess is exactly zero, so the first residual
 // r = b - A*0 is b itself and its matrix product is skipped.
-// With Z (para->bcg_precond), the CG is preconditioned by the diagonal of A:
-// z = D^-1 r, rho = r.z drives alpha and beta and p starts from z, while the
-// stopping rule keeps the reference's ||r_k|| / ||r_0|| (e2 = r.r).
+// With para->bcg_precond the CG is preconditioned by the diagonal D of A:
+// rho = r.(D^-1 r) drives alpha and beta and p starts from D^-1 r (formed by
+// the search step, never stored), while the stopping rule keeps the
+// reference's ||r_k|| / ||r_0|| (e2 = r.r).
 void bcg(void *A, void **RHS, void **V, int x_start, int x_length, GCGE_OPS *ops, GCGE_PARA *para, void **CG_P,
-         void **CG_W, double *dtmp, int *itmp, bool x_is_zero = false, void **Z = nullptr)
+         void **CG_W, double *dtmp, int *itmp, bool x_is_zero = false)
 {
     if (x_length <= 0) return;
-    const bool pre = Z != nullptr;
+    const bool pre = para->bcg_precond != 0;
     std::vector<double> e2(pre ? x_length : 0);
     double *rho1 = dtmp, *rho2 = rho1 + x_length, *error = rho2 + x_length;
     double *last_error = error + x_length, *ptw = last_error + x_length;
@@ -580,14 +581,13 @@ void bcg(void *A, void **RHS, void **V, int x_start, int x_length, GCGE_OPS *ops
     if (pre) {
         // alpha = 0 leaves r as it is (the a == 0 branch of y = a x + b y)
         std::vector<double> zero(x_length, 0.0);
-        ops->BlockPCGResidualUpdate(A, CG_W, RHS, Z, x_length, zero.data(), rho2, e2.data(), ops);
+        ops->BlockPCGResidualUpdate(A, CG_W, RHS, x_length, zero.data(), rho2, e2.data(), ops);
     } else {
         ops->MultiVecInnerProdDiag(RHS, RHS, rho2, ms, me, ops);
     }
     for (int idx = 0; idx < x_length; idx++) unlock[idx] = idx;
     int num_unlock = x_length;
     for (int idx = 0; idx < x_length; idx++) error[idx] = std::sqrt(pre ? e2[idx] : rho2[idx]);
-    void **S = pre ? Z : RHS;  // the vector the search direction is built from
     // The solution update x += alpha p of a step is deferred into the search
     // direction update of the next step (p is read there before it is
     // overwritten); every element sees the reference's roundings in the
@@ -598,10 +598,17 @@ void bcg(void *A, void **RHS, void **V, int x_start, int x_length, GCGE_OPS *ops
         set2(ms, 0, 0);
         set2(me, na, na);
         if (niter == 1) {
-            ops->MultiVecAxpby(1.0, S, 0.0, CG_P, ms, me, ops);
+            if (pre) {
+                // p = D^-1 r (beta 0: the copy branch), nothing to add to x yet
+                std::fill(b.begin(), b.end(), 0.0);
+                ops->BlockPCGSearchUpdate(A, V, x_start, CG_P, RHS, 0, na, alpha.data(), b.data(), ops);
+            } else {
+                ops->MultiVecAxpby(1.0, RHS, 0.0, CG_P, ms, me, ops);
+            }
         } else {
             for (int id = 0; id < na; id++) b[id] = rho2[id] / rho1[id];
-            ops->BlockCGSearchUpdate(V, x_start, CG_P, S, na_prev, na, alpha.data(), b.data(), ops);
+            if (pre) ops->BlockPCGSearchUpdate(A, V, x_start, CG_P, RHS, na_prev, na, alpha.data(), b.data(), ops);
+            else ops->BlockCGSearchUpdate(V, x_start, CG_P, RHS, na_prev, na, alpha.data(), b.data(), ops);
         }
         ops->MatDotMultiVecInnerProdDiag(A, CG_P, CG_W, ms, me, ptw, ops);
         for (int id = 0; id < na; id++) {
@@ -610,7 +617,7 @@ void bcg(void *A, void **RHS, void **V, int x_start, int x_length, GCGE_OPS *ops
         }
         if (pre) {
             std::vector<double> rz(na);
-            ops->BlockPCGResidualUpdate(A, CG_W, RHS, Z, na, alpha.data(), rz.data(), rr.data(), ops);
+            ops->BlockPCGResidualUpdate(A, CG_W, RHS, na, alpha.data(), rz.data(), rr.data(), ops);
             for (int id = 0; id < na; id++) {
                 rho2[id] = rz[id];
                 e2[id] = rr[id];
@@ -631,7 +638,7 @@ void bcg(void *A, void **RHS, void **V, int x_start, int x_length, GCGE_OPS *ops
         }
         niter++;
     }
-    if (na_prev > 0) ops->BlockCGSearchUpdate(V, x_start, CG_P, S, na_prev, 0, alpha.data(), b.data(), ops);
+    if (na_prev > 0) ops->BlockCGSearchUpdate(V, x_start, CG_P, RHS, na_prev, 0, alpha.data(), b.data(), ops);
     para->bcg_calls++;
     para->bcg_iters_total += niter - 1;
 }
@@ -667,27 +674,12 @@ bool bcg32_ready(void *A, int cols, GCGE_OPS *ops, GCGE_WORKSPACE *ws)
     return ws->bcg32[0] && ws->bcg32[1] && ws->bcg32[2] && ws->bcg32[3];
 }
 
-// z columns of the preconditioned block CG (para->bcg_precond), in the basis
-// precision or in FP32 for the FP32 inner solves; created once per workspace.
-void **pcg_z_ready(void *A, int cols, bool fp32, GCGE_OPS *ops, GCGE_WORKSPACE *ws)
+// the preconditioned block CG needs its two slots
+bool pcg_ready(GCGE_OPS *ops, GCGE_PARA *para)
 {
-    if (!ops->BlockPCGResidualUpdate) {
-        fprintf(stderr, "GCGE: bcg_precond 1 needs the BlockPCGResidualUpdate slot\n");
-        return nullptr;
-    }
-    void ***z = fp32 ? &ws->pcg_z32 : &ws->pcg_z;
-    GCGE_INT *nc = fp32 ? &ws->pcg_z32_cols : &ws->pcg_z_cols;
-    if (*z && *nc >= cols) return *z;
-    if (*z) ops->MultiVecDestroy(z, *nc, ops);
-    const int n = ws->CG_p_size > cols ? ws->CG_p_size : cols;
-    if (fp32) {
-        if (!ops->MultiVecCreateByMatPrecision) return nullptr;
-        ops->MultiVecCreateByMatPrecision(z, n, A, 1, ops);
-    } else {
-        ops->MultiVecCreateByMat(z, n, A, ops);
-    }
-    *nc = n;
-    return *z;
+    if (!para->bcg_precond || (ops->BlockPCGResidualUpdate && ops->BlockPCGSearchUpdate)) return true;
+    fprintf(stderr, "GCGE: bcg_precond 1 needs the BlockPCGResidualUpdate and BlockPCGSearchUpdate slots\n");
+    return false;
 }
 
 // GCGE_ComputeW, gcge_xpw.c:118-217: W_j starts from x_j (|lambda| <= 1) or
@@ -739,9 +731,7 @@ void compute_w(void *A, void *B, void **V, double *eval, GCGE_OPS *ops, GCGE_PAR
         // copies (the same CG iterates in exact arithmetic, and the same
         // ||r_k|| / ||r_0|| stopping rule), then W = W0 + E in FP64.  FP32
         // rounding is relative to the correction, which shrinks as X converges.
-        if (!bcg32_ready(A, w_length, ops, ws)) return;
-        void **Z32 = nullptr;
-        if (para->bcg_precond && !(Z32 = pcg_z_ready(A, w_length, true, ops, ws))) return;
+        if (!bcg32_ready(A, w_length, ops, ws) || !pcg_ready(ops, para)) return;
         set2(ms, w_start, 0);
         set2(me, w_start + w_length, w_length);
         ops->MatDotMultiVec(A, V, ws->evec, ms, me, ops);
@@ -751,16 +741,14 @@ void compute_w(void *A, void *B, void **V, double *eval, GCGE_OPS *ops, GCGE_PAR
         ops->MultiVecCopy(ws->V_tmp, ws->bcg32[0], ms, me, ops);
         ops->MultiVecAxpby(0.0, ws->bcg32[1], 0.0, ws->bcg32[1], ms, me, ops);
         bcg(A, ws->bcg32[0], ws->bcg32[1], 0, w_length, ops, para, ws->bcg32[2], ws->bcg32[3], ws->subspace_dtmp,
-            ws->subspace_itmp, true, Z32);
+            ws->subspace_itmp, true);
         ops->MultiVecCopy(ws->bcg32[1], ws->V_tmp, ms, me, ops);
         set2(ms, 0, w_start);
         set2(me, w_length, w_start + w_length);
         ops->MultiVecAxpby(1.0, ws->V_tmp, 1.0, V, ms, me, ops);
     } else if (para->if_use_bcg == 1) {
-        void **Z = nullptr;
-        if (para->bcg_precond && !(Z = pcg_z_ready(A, w_length, false, ops, ws))) return;
-        bcg(A, ws->V_tmp, V, w_start, w_length, ops, para, ws->CG_p, ws->evec, ws->subspace_dtmp, ws->subspace_itmp,
-            false, Z);
+        if (!pcg_ready(ops, para)) return;
+        bcg(A, ws->V_tmp, V, w_start, w_length, ops, para, ws->CG_p, ws->evec, ws->subspace_dtmp, ws->subspace_itmp);
     } else {
         for (int idx = 0; idx < w_length; idx++) {
             void *rhs, *wc;
@@ -1255,8 +1243,6 @@ extern "C" void GCGE_WORKSPACE_Free(GCGE_WORKSPACE **workspace, GCGE_PARA *, GCG
     }
     for (int i = 0; i < 4; i++)
         if (ws->bcg32[i]) ops->MultiVecDestroy(&ws->bcg32[i], ws->bcg32_cols, ops);
-    if (ws->pcg_z) ops->MultiVecDestroy(&ws->pcg_z, ws->pcg_z_cols, ops);
-    if (ws->pcg_z32) ops->MultiVecDestroy(&ws->pcg_z32, ws->pcg_z32_cols, ops);
     free(ws);
     *workspace = nullptr;
 }

@@ -975,20 +975,39 @@ void s_BlockCGResidualUpdate(void **w, void **r, GCGE_INT k, const GCGE_DOUBLE *
         for (int j = 0; j < k; j++) rr[j] = NAN;
 }
 
-void s_BlockPCGResidualUpdate(void *Mat, void **w, void **r, void **z, GCGE_INT k, const GCGE_DOUBLE *alpha,
-                              GCGE_DOUBLE *rz, GCGE_DOUBLE *rr, GCGE_OPS *)
+void s_BlockPCGResidualUpdate(void *Mat, void **w, void **r, GCGE_INT k, const GCGE_DOUBLE *alpha, GCGE_DOUBLE *rz,
+                              GCGE_DOUBLE *rr, GCGE_OPS *)
 {
     HIP_MAT *A = (HIP_MAT *)Mat;
-    MV *mw = as_mv(w), *mr = as_mv(r), *mz = as_mv(z);
+    MV *mw = as_mv(w), *mr = as_mv(r);
     const char *who = "BlockPCGResidualUpdate";
-    if (!range_ok(mw, 0, k, who) || !range_ok(mr, 0, k, who) || !range_ok(mz, 0, k, who) ||
-        !same_rows(mw, mr, who) || !same_rows(mr, mz, who) || !same_prec(mw, mr, who) || !same_prec(mr, mz, who) ||
+    if (!range_ok(mw, 0, k, who) || !range_ok(mr, 0, k, who) || !same_rows(mw, mr, who) || !same_prec(mw, mr, who) ||
         !mat_ok(A, mr, who))
         return;
     const void *dinv = mat_dinv(A, mr->prec);
-    if (!dinv || !pcg_resid(mr->prec, mr->n, mw->data, mw->ld, mr->data, mr->ld, mz->data, mz->ld, dinv, k, alpha,
-                            rz, rr))
+    if (!dinv || !pcg_resid(mr->prec, mr->n, mw->data, mw->ld, mr->data, mr->ld, dinv, k, alpha, rz, rr))
         for (int j = 0; j < k; j++) rz[j] = rr[j] = NAN;
+}
+
+void s_BlockPCGSearchUpdate(void *Mat, void **x, GCGE_INT sx, void **p, void **r, GCGE_INT kx, GCGE_INT kp,
+                            const GCGE_DOUBLE *alpha, const GCGE_DOUBLE *beta, GCGE_OPS *)
+{
+    HIP_MAT *A = (HIP_MAT *)Mat;
+    MV *mx = as_mv(x), *mp = as_mv(p), *mr = as_mv(r);
+    const char *who = "BlockPCGSearchUpdate";
+    if (kp < 0 || kx < 0) {
+        latch(172, "%s: negative column count (kx %d, kp %d)", who, kx, kp);
+        return;
+    }
+    const int k = std::max(kx, kp);
+    if (!range_ok(mx, sx, sx + kx, who) || !range_ok(mp, 0, k, who) || !range_ok(mr, 0, kp, who) ||
+        !same_rows(mx, mp, who) || !same_rows(mx, mr, who) || !same_prec(mx, mp, who) || !same_prec(mx, mr, who) ||
+        !mat_ok(A, mr, who))
+        return;
+    const void *dinv = mat_dinv(A, mr->prec);
+    if (!dinv) return;
+    launch_cg_search(mx->prec, mx->n, col_ptr(mx, sx), mx->ld, mp->data, mp->ld, mr->data, mr->ld, kx, kp, alpha,
+                     beta, dinv);
 }
 
 void s_BlockCGSearchUpdate(void **x, GCGE_INT sx, void **p, void **r, GCGE_INT kx, GCGE_INT kp,
@@ -1123,6 +1142,7 @@ extern "C" void GCGE_HIP_SetOps(GCGE_OPS *ops)
     ops->BlockCGResidualUpdate = s_BlockCGResidualUpdate;
     ops->BlockCGSearchUpdate = s_BlockCGSearchUpdate;
     ops->BlockPCGResidualUpdate = s_BlockPCGResidualUpdate;
+    ops->BlockPCGSearchUpdate = s_BlockPCGSearchUpdate;
     ops->MultiVecResidualNorms = s_MultiVecResidualNorms;
     ops->MultiVecGather = s_MultiVecGather;
     ops->MultiVecCopy = s_MultiVecCopy;

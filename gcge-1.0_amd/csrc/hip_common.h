@@ -165,14 +165,15 @@ bool cg_update(int prec, int64_t n, void *x, int64_t ldx, const void *p, int64_t
 // r_j -= alpha_j w_j ; rr_j = <r_j, r_j>
 bool cg_resid(int prec, int64_t n, const void *w, int64_t ldw, void *r, int64_t ldr, int k, const double *alpha,
               double *rr);
-// r_j -= alpha_j w_j ; z_j = dinv .* r_j ; rz_j = <r_j, z_j> ; rr_j = <r_j, r_j>
-bool pcg_resid(int prec, int64_t n, const void *w, int64_t ldw, void *r, int64_t ldr, void *z, int64_t ldz,
-               const void *dinv, int k, const double *alpha, double *rz, double *rr);
+// r_j -= alpha_j w_j ; rz_j = <r_j, dinv .* r_j> ; rr_j = <r_j, r_j>
+bool pcg_resid(int prec, int64_t n, const void *w, int64_t ldw, void *r, int64_t ldr, const void *dinv, int k,
+               const double *alpha, double *rz, double *rr);
 // dinv64[i] = 1 / a_ii (1 where a_ii is 0 or absent), dinv32 its FP32 rounding
 void launch_diag_inv(HIP_MAT *A);
-// x_j += alpha_j p_j (j < kx), then p_j = r_j + beta_j p_j (j < kp)
+// x_j += alpha_j p_j (j < kx), then p_j = r_j + beta_j p_j (j < kp); with
+// dinv (not null), p_j = dinv .* r_j + beta_j p_j
 void launch_cg_search(int prec, int64_t n, void *x, int64_t ldx, void *p, int64_t ldp, const void *r, int64_t ldr,
-                      int kx, int kp, const double *alpha, const double *beta);
+                      int kx, int kp, const double *alpha, const double *beta, const void *dinv = nullptr);
 // G (p x q, host, column-major ldg) = V^T W
 bool gram(int prec, int64_t n, const void *V, int64_t ldv, int p, const void *W, int64_t ldw, int q,
           double *G, int ldg, bool upper_only);

@@ -471,26 +471,24 @@ __global__ __launch_bounds__(kBlock) void k_cg_resid(int64_t n, const T *__restr
 }
 
 // Jacobi-preconditioned residual step: r -= alpha w (same rounding as
-// k_cg_resid), z = dinv .* r, partial r.z (row 2j) and r.r (row 2j + 1)
+// k_cg_resid), partial r.z (row 2j) and r.r (row 2j + 1) with z = dinv .* r
+// rounded as k_cg_search<T, true> forms it
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_pcg_resid(int64_t n, const T *__restrict__ w, int64_t ldw,
-                                                      T *__restrict__ r, int64_t ldr, T *__restrict__ z,
-                                                      int64_t ldz, const T *__restrict__ dinv, ColScalars s,
-                                                      double *__restrict__ partial)
+                                                      T *__restrict__ r, int64_t ldr, const T *__restrict__ dinv,
+                                                      ColScalars s, double *__restrict__ partial)
 {
     __shared__ double sh[4];
     const int j = blockIdx.y;
     const T al = (T)s.a[j];
     const T *wj = w + (int64_t)j * ldw;
     T *rj = r + (int64_t)j * ldr;
-    T *zj = z + (int64_t)j * ldz;
     double arz = 0.0, arr = 0.0;
     const int64_t stride = (int64_t)gridDim.x * kBlock;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
         const T rn = axpby_el<T>(-al, wj[i], T(1), rj[i]);
         const T zn = rn * dinv[i];
         rj[i] = rn;
-        zj[i] = zn;
         arz = fma((double)rn, (double)zn, arz);
         arr = fma((double)rn, (double)rn, arr);
     }
@@ -519,10 +517,13 @@ __global__ __launch_bounds__(kBlock) void k_diag_inv(int n, const int *__restric
 
 // x += alpha p (VecAxpby(alpha, p, 1.0, x)) for the columns j < kx; then
 // p = r + beta p (MultiVecAxpbyDiag(1, r, beta, p)) for the columns j < kp
-template <typename T>
+// With PRE, the search direction is built from z = dinv .* r (the Jacobi-
+// preconditioned residual, formed here rather than stored).
+template <typename T, bool PRE>
 __global__ __launch_bounds__(kBlock) void k_cg_search(int64_t n, T *__restrict__ x, int64_t ldx,
                                                       T *__restrict__ p, int64_t ldp, const T *__restrict__ r,
-                                                      int64_t ldr, int kx, int kp, ColScalars s)
+                                                      int64_t ldr, int kx, int kp, ColScalars s,
+                                                      const T *__restrict__ dinv)
 {
     const int j = blockIdx.y;
     const T al = (T)s.a[j], be = (T)s.b[j];
@@ -535,14 +536,14 @@ __global__ __launch_bounds__(kBlock) void k_cg_search(int64_t n, T *__restrict__
         for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
             const T pv = pj[i];
             xj[i] = axpby_el<T>(al, pv, T(1), xj[i]);
-            pj[i] = axpby_el<T>(T(1), rj[i], be, pv);
+            pj[i] = axpby_el<T>(T(1), PRE ? rj[i] * dinv[i] : rj[i], be, pv);
         }
     } else if (dx) {
         for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
             xj[i] = axpby_el<T>(al, pj[i], T(1), xj[i]);
     } else if (dp) {
         for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
-            pj[i] = axpby_el<T>(T(1), rj[i], be, pj[i]);
+            pj[i] = axpby_el<T>(T(1), PRE ? rj[i] * dinv[i] : rj[i], be, pj[i]);
     }
 }
 
@@ -867,8 +868,8 @@ bool cg_resid(int prec, int64_t n, const void *w, int64_t ldw, void *r, int64_t 
     return true;
 }
 
-bool pcg_resid(int prec, int64_t n, const void *w, int64_t ldw, void *r, int64_t ldr, void *z, int64_t ldz,
-               const void *dinv, int k, const double *alpha, double *rz, double *rr)
+bool pcg_resid(int prec, int64_t n, const void *w, int64_t ldw, void *r, int64_t ldr, const void *dinv, int k,
+               const double *alpha, double *rz, double *rr)
 {
     if (k <= 0) return true;
     if (failed()) return false;
@@ -882,15 +883,15 @@ bool pcg_resid(int prec, int64_t n, const void *w, int64_t ldw, void *r, int64_t
         dim3 grid(nb, kc);
         {
             ProfScope ps(prec == GCGE_HIP_FP64 ? K_CG_UPDATE_F64 : K_CG_UPDATE_F32,
-                         5.0 * kc * (double)n * elem_size(prec), 6.0 * kc * (double)n);
+                         3.0 * kc * (double)n * elem_size(prec) + (double)n * elem_size(prec), 6.0 * kc * (double)n);
             if (prec == GCGE_HIP_FP64)
                 k_pcg_resid<double><<<grid, kBlock, 0, ctx().stream>>>(
                     n, (const double *)w + (int64_t)c0 * ldw, ldw, (double *)r + (int64_t)c0 * ldr, ldr,
-                    (double *)z + (int64_t)c0 * ldz, ldz, (const double *)dinv, s, partial);
+                    (const double *)dinv, s, partial);
             else
                 k_pcg_resid<float><<<grid, kBlock, 0, ctx().stream>>>(
                     n, (const float *)w + (int64_t)c0 * ldw, ldw, (float *)r + (int64_t)c0 * ldr, ldr,
-                    (float *)z + (int64_t)c0 * ldz, ldz, (const float *)dinv, s, partial);
+                    (const float *)dinv, s, partial);
         }
         if (!check(hipGetLastError(), "k_pcg_resid")) return false;
         if (!reduce_to_host(partial, 2 * kc, nb, tmp.data())) return false;
@@ -910,7 +911,7 @@ void launch_diag_inv(HIP_MAT *A)
 }
 
 void launch_cg_search(int prec, int64_t n, void *x, int64_t ldx, void *p, int64_t ldp, const void *r, int64_t ldr,
-                      int kx, int kp, const double *alpha, const double *beta)
+                      int kx, int kp, const double *alpha, const double *beta, const void *dinv)
 {
     const int k = std::max(kx, kp);
     if (k <= 0 || failed()) return;
@@ -925,15 +926,22 @@ void launch_cg_search(int prec, int64_t n, void *x, int64_t ldx, void *p, int64_
         dim3 grid(nb, kc);
         {
             ProfScope ps(prec == GCGE_HIP_FP64 ? K_CG_SEARCH_F64 : K_CG_SEARCH_F32,
-                         (3.0 * kxc + 3.0 * kpc - 1.0 * std::min(kxc, kpc)) * (double)n * elem_size(prec), 0.0);
-            if (prec == GCGE_HIP_FP64)
-                k_cg_search<double><<<grid, kBlock, 0, ctx().stream>>>(
-                    n, (double *)x + (int64_t)c0 * ldx, ldx, (double *)p + (int64_t)c0 * ldp, ldp,
-                    (const double *)r + (int64_t)c0 * ldr, ldr, kxc, kpc, s);
-            else
-                k_cg_search<float><<<grid, kBlock, 0, ctx().stream>>>(
-                    n, (float *)x + (int64_t)c0 * ldx, ldx, (float *)p + (int64_t)c0 * ldp, ldp,
-                    (const float *)r + (int64_t)c0 * ldr, ldr, kxc, kpc, s);
+                         (3.0 * kxc + 3.0 * kpc - 1.0 * std::min(kxc, kpc)) * (double)n * elem_size(prec) +
+                             (dinv && kpc > 0 ? (double)n * elem_size(prec) : 0.0),
+                         0.0);
+#define GH_SEARCH(T_, PRE_)                                                                                     \
+    k_cg_search<T_, PRE_><<<grid, kBlock, 0, ctx().stream>>>(n, (T_ *)x + (int64_t)c0 * ldx, ldx,              \
+                                                              (T_ *)p + (int64_t)c0 * ldp, ldp,              \
+                                                              (const T_ *)r + (int64_t)c0 * ldr, ldr, kxc, kpc, \
+                                                              s, (const T_ *)dinv)
+            if (prec == GCGE_HIP_FP64) {
+                if (dinv) GH_SEARCH(double, true);
+                else GH_SEARCH(double, false);
+            } else {
+                if (dinv) GH_SEARCH(float, true);
+                else GH_SEARCH(float, false);
+            }
+#undef GH_SEARCH
         }
         if (!check(hipGetLastError(), "k_cg_search")) return;
     }

@@ -231,13 +231,20 @@ class Ops:
         _check("BlockCGResidualUpdate")
         return rr
 
-    def pcg_residual_update(self, A: Matrix, W: MultiVec, R: MultiVec, Z: MultiVec, k, alpha):
+    def pcg_residual_update(self, A: Matrix, W: MultiVec, R: MultiVec, k, alpha):
         alpha = np.ascontiguousarray(alpha, dtype=np.float64)
         rz, rr = np.zeros(k), np.zeros(k)
-        self._f("BlockPCGResidualUpdate", capi.F_BlockPCGResidualUpdate)(A.ptr, W.ptr, R.ptr, Z.ptr, k, _dp(alpha),
+        self._f("BlockPCGResidualUpdate", capi.F_BlockPCGResidualUpdate)(A.ptr, W.ptr, R.ptr, k, _dp(alpha),
                                                                          _dp(rz), _dp(rr), self._p)
         _check("BlockPCGResidualUpdate")
         return rz, rr
+
+    def pcg_search_update(self, A: Matrix, X: MultiVec, xs, P: MultiVec, R: MultiVec, kx, kp, alpha, beta):
+        alpha = np.ascontiguousarray(alpha, dtype=np.float64)
+        beta = np.ascontiguousarray(beta, dtype=np.float64)
+        self._f("BlockPCGSearchUpdate", capi.F_BlockPCGSearchUpdate)(A.ptr, X.ptr, xs, P.ptr, R.ptr, kx, kp,
+                                                                     _dp(alpha), _dp(beta), self._p)
+        _check("BlockPCGSearchUpdate")
 
     def cg_search_update(self, X: MultiVec, xs, P: MultiVec, R: MultiVec, kx, kp, alpha, beta):
         alpha = np.ascontiguousarray(alpha, dtype=np.float64)

@@ -80,7 +80,6 @@ class GCGE_WORKSPACE(C.Structure):
         ("num_soft_locked_in_last_iter", c_int), ("dense_matrix", c_void_p),
         ("subspace_matrix", DP), ("subspace_evec", DP), ("subspace_dtmp", DP), ("subspace_itmp", IP),
         ("eval", DP), ("orth_ind", IP), ("bcg32", c_void_p * 4), ("bcg32_cols", c_int),
-        ("pcg_z", c_void_p), ("pcg_z32", c_void_p), ("pcg_z_cols", c_int), ("pcg_z32_cols", c_int),
     ]
 
 
@@ -97,6 +96,7 @@ OPS_SLOTS = [
     "MultiVecAxpbyDiag", "MultiVecInnerProdDiag", "MultiVecLinearCombAxpby", "MatDotMultiVecInnerProdDiag",
     "BlockCGUpdate", "MultiVecResidualNorms", "MultiVecGather", "MultiVecCopy", "MultiVecNumRows",
     "MultiVecCreateByMatPrecision", "BlockCGResidualUpdate", "BlockCGSearchUpdate", "BlockPCGResidualUpdate",
+    "BlockPCGSearchUpdate",
 ]
 
 
@@ -203,7 +203,9 @@ F_MatDotMultiVecInnerProdDiag = C.CFUNCTYPE(None, c_void_p, c_void_p, c_void_p, 
 F_BlockCGUpdate = C.CFUNCTYPE(None, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int, DP, DP, c_void_p)
 F_BlockCGResidualUpdate = C.CFUNCTYPE(None, c_void_p, c_void_p, c_int, DP, DP, c_void_p)
 F_BlockCGSearchUpdate = C.CFUNCTYPE(None, c_void_p, c_int, c_void_p, c_void_p, c_int, c_int, DP, DP, c_void_p)
-F_BlockPCGResidualUpdate = C.CFUNCTYPE(None, c_void_p, c_void_p, c_void_p, c_void_p, c_int, DP, DP, DP, c_void_p)
+F_BlockPCGResidualUpdate = C.CFUNCTYPE(None, c_void_p, c_void_p, c_void_p, c_int, DP, DP, DP, c_void_p)
+F_BlockPCGSearchUpdate = C.CFUNCTYPE(None, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_int, DP, DP,
+                                     c_void_p)
 F_MultiVecResidualNorms = C.CFUNCTYPE(None, c_void_p, c_void_p, c_void_p, IP, c_int, DP, DP, DP, DP, c_void_p)
 F_MultiVecGather = C.CFUNCTYPE(None, c_void_p, IP, c_int, c_void_p, c_int, c_void_p)
 F_MultiVecCopy = C.CFUNCTYPE(None, c_void_p, c_void_p, IP, IP, c_void_p)

@@ -144,13 +144,18 @@ typedef struct GCGE_OPS_ {
      * j < kp, per element in that order (kx and kp in any relation) */
     void (*BlockCGSearchUpdate)(void **x, GCGE_INT sx, void **p, void **r, GCGE_INT kx, GCGE_INT kp,
                                 const GCGE_DOUBLE *alpha, const GCGE_DOUBLE *beta, struct GCGE_OPS_ *ops);
-    /* residual step of the Jacobi-preconditioned block CG (para->bcg_precond;
-     * the reference's BCG has no preconditioner): r(:,j) -= alpha[j] w(:,j)
-     * as in BlockCGResidualUpdate, z(:,j) = D^-1 r(:,j) with D the diagonal of
-     * Matrix, rz[j] = <r(:,j), z(:,j)>, rr[j] = <r(:,j), r(:,j)>, j < k.
-     * Optional: needed only when bcg_precond is set. */
-    void (*BlockPCGResidualUpdate)(void *Matrix, void **w, void **r, void **z, GCGE_INT k, const GCGE_DOUBLE *alpha,
+    /* the two steps of the Jacobi-preconditioned block CG (para->bcg_precond;
+     * the reference's BCG has no preconditioner), with z = D^-1 r formed where
+     * it is used (D the diagonal of Matrix) instead of stored.  Optional:
+     * needed only when bcg_precond is set.
+     * Residual step: r(:,j) -= alpha[j] w(:,j) as in BlockCGResidualUpdate,
+     * rz[j] = <r(:,j), D^-1 r(:,j)>, rr[j] = <r(:,j), r(:,j)>, j < k. */
+    void (*BlockPCGResidualUpdate)(void *Matrix, void **w, void **r, GCGE_INT k, const GCGE_DOUBLE *alpha,
                                    GCGE_DOUBLE *rz, GCGE_DOUBLE *rr, struct GCGE_OPS_ *ops);
+    /* Search step: as BlockCGSearchUpdate with p(:,j) = D^-1 r(:,j) + beta[j] p(:,j) */
+    void (*BlockPCGSearchUpdate)(void *Matrix, void **x, GCGE_INT sx, void **p, void **r, GCGE_INT kx,
+                                 GCGE_INT kp, const GCGE_DOUBLE *alpha, const GCGE_DOUBLE *beta,
+                                 struct GCGE_OPS_ *ops);
 } GCGE_OPS;
 
 void GCGE_OPS_Create(GCGE_OPS **ops);

@@ -63,10 +63,6 @@ typedef struct GCGE_WORKSPACE_ {
     /* ---- members of this build (no counterpart in the reference) ---- */
     void **bcg32[4];     /* FP32 block-CG work columns (rhs/residual, solution, p, w) */
     GCGE_INT bcg32_cols; /* columns allocated in each of them                        */
-    void **pcg_z;        /* preconditioned residual z of the block CG, basis precision */
-    void **pcg_z32;      /* the same for the FP32 inner solves                          */
-    GCGE_INT pcg_z_cols; /* columns allocated in pcg_z                                  */
-    GCGE_INT pcg_z32_cols; /* columns allocated in pcg_z32                              */
 } GCGE_WORKSPACE;
 
 typedef struct GCGE_SOLVER_ {

@@ -377,42 +377,49 @@ def test_cg_split_steps_bit_exact(ops):
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
-def test_pcg_residual_update(ops, prec):
-    """BlockPCGResidualUpdate: r as BlockCGResidualUpdate's (bit for bit),
-    z = r * (1 / a_ii) with 1 / a_ii rounded once in FP64 (then to FP32 for
-    FP32 data), and r.z, r.r accumulated in FP64; a zero alpha leaves r as
-    it is.  The matrix has a row without a stored diagonal entry (z = r
-    there)."""
+def test_pcg_steps(ops, prec):
+    """The Jacobi-preconditioned block CG steps, z = D^-1 r formed where it
+    is used: BlockPCGResidualUpdate leaves r as BlockCGResidualUpdate does
+    (bit for bit) and returns r.z and r.r (FP64 accumulation);
+    BlockPCGSearchUpdate adds alpha p to x and sets p = z + beta p with the
+    same z, 1 / a_ii rounded once in FP64 (then to FP32 for FP32 data).  A
+    zero alpha leaves r as it is; a row without a stored diagonal entry is
+    not scaled."""
     A = MATS["vardiff2d"].tolil()
     A[5, 5] = 0.0
     A = A.tocsr()
     A.eliminate_zeros()
-    n, k = A.shape[0], 7
+    n, k, kx = A.shape[0], 7, 4
     g = rng(44)
-    W, R = g.standard_normal((n, k)), g.standard_normal((n, k))
-    al = g.standard_normal(k)
+    W, R, X, P_ = (g.standard_normal((n, k)) for _ in range(4))
+    al, be = g.standard_normal(k), g.standard_normal(k)
+    be[1] = 0.0
     f32 = prec == "f32"
+    T = np.float32 if f32 else np.float64
     pr = F32 if f32 else capi.GCGE_HIP_FP64
     mA = Matrix(A)
-    w, r, z = MultiVec(mA, k, pr, data=W), MultiVec(mA, k, pr, data=R), MultiVec(mA, k, pr)
+    w, r, x, p = (MultiVec(mA, k, pr, data=M) for M in (W, R, X, P_))
+    W, R, X, P_ = (M.astype(T) for M in (W, R, X, P_))
     d = A.diagonal()
-    dinv = np.where(d != 0.0, 1.0 / np.where(d != 0.0, d, 1.0), 1.0)
-    if f32:
-        W, R = W.astype(np.float32), R.astype(np.float32)
-        r_ref = (-al.astype(np.float32))[None, :] * W + R
-        z_ref = r_ref * dinv.astype(np.float32)[:, None]
-    else:
-        r_ref = (-al)[None, :] * W + R
-        z_ref = r_ref * dinv[:, None]
-    rz, rr = ops.pcg_residual_update(mA, w, r, z, k, al)
+    dinv = np.where(d != 0.0, 1.0 / np.where(d != 0.0, d, 1.0), 1.0).astype(T)
+    r_ref = (-al.astype(T))[None, :] * W + R
+    z_ref = r_ref * dinv[:, None]
+    rz, rr = ops.pcg_residual_update(mA, w, r, k, al)
     np.testing.assert_array_equal(r.get(), r_ref.astype(np.float64))
-    np.testing.assert_array_equal(z.get(), z_ref.astype(np.float64))
     rd, zd = r_ref.astype(np.float64), z_ref.astype(np.float64)
     np.testing.assert_allclose(rz, np.einsum("ij,ij->j", rd, zd), rtol=RTOL_RED)
     np.testing.assert_allclose(rr, np.einsum("ij,ij->j", rd, rd), rtol=RTOL_RED)
-    rz0, rr0 = ops.pcg_residual_update(mA, w, r, z, k, np.zeros(k))
+    rz0, rr0 = ops.pcg_residual_update(mA, w, r, k, np.zeros(k))
     np.testing.assert_array_equal(r.get(), rd)
     np.testing.assert_allclose(rr0, rr, rtol=0)
+    np.testing.assert_allclose(rz0, rz, rtol=0)
+    ops.pcg_search_update(mA, x, 0, p, r, kx, k, al, be)
+    x_ref = X.copy()
+    x_ref[:, :kx] = al[None, :kx].astype(T) * P_[:, :kx] + X[:, :kx]
+    p_ref = z_ref + be.astype(T)[None, :] * P_
+    p_ref[:, 1] = z_ref[:, 1]  # beta == 0: p = z (copy branch)
+    np.testing.assert_array_equal(x.get(), x_ref.astype(np.float64))
+    np.testing.assert_array_equal(p.get(), p_ref.astype(np.float64))
 
 
 @pytest.mark.parametrize("withB", [False, True])
