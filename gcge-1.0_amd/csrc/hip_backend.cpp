@@ -412,6 +412,10 @@ extern "C" HIP_MAT *HIP_MatCreateFromCSR(GCGE_INT nrows, GCGE_INT ncols, GCGE_IN
     A->prec = GCGE_HIP_FP64;
     A->max_row_nnz = maxrow;
     A->row_abs_max = rowabs;
+    std::vector<int> longs;
+    for (int i = 0; i < nrows; i++)
+        if (rowptr[i + 1] - rowptr[i] > kLongRow) longs.push_back(i);
+    A->nlong = (int)longs.size();
     bool ok = check(hipMalloc((void **)&A->rowptr, sizeof(int) * (nrows + 1)), "hipMalloc rowptr") &&
               check(hipMalloc((void **)&A->col, sizeof(int) * std::max(1, nnz)), "hipMalloc col") &&
               check(hipMalloc((void **)&A->val64, sizeof(double) * std::max(1, nnz)), "hipMalloc val");
@@ -420,6 +424,10 @@ extern "C" HIP_MAT *HIP_MatCreateFromCSR(GCGE_INT nrows, GCGE_INT ncols, GCGE_IN
              (nnz == 0 || check(hipMemcpy(A->col, col, sizeof(int) * nnz, hipMemcpyHostToDevice), "H2D col")) &&
              (nnz == 0 || check(hipMemcpy(A->val64, val, sizeof(double) * nnz, hipMemcpyHostToDevice), "H2D val"));
     }
+    if (ok && A->nlong > 0)
+        ok = check(hipMalloc((void **)&A->long_rows, sizeof(int) * A->nlong), "hipMalloc long_rows") &&
+             check(hipMemcpy(A->long_rows, longs.data(), sizeof(int) * A->nlong, hipMemcpyHostToDevice),
+                   "H2D long_rows");
     if (!ok) {
         HIP_MatDestroy(&A);
         return nullptr;
@@ -467,6 +475,7 @@ extern "C" void HIP_MatDestroy(HIP_MAT **mat)
         if (A->val32) (void)hipFree(A->val32);
         if (A->rowidx) (void)hipFree(A->rowidx);
         if (A->dinv64) (void)hipFree(A->dinv64);
+        if (A->long_rows) (void)hipFree(A->long_rows);
         if (A->dinv32) (void)hipFree(A->dinv32);
         g_ctx.dev_bytes -= sizeof(int) * (double)(A->n + 1) + 12.0 * A->nnz + (A->val32 ? 4.0 * A->nnz : 0.0) +
                            (A->rowidx ? 4.0 * A->nnz : 0.0);

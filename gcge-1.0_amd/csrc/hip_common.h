@@ -46,6 +46,11 @@ struct MV {
 
 }  // namespace gh
 
+// Rows with more than this many stored entries are multiplied by a whole
+// wavefront each (k_spmm_long) instead of one thread (k_spmm): power-law
+// graphs carry rows thousands of entries long.
+constexpr int kLongRow = 64;
+
 struct HIP_MAT_ {
     uint32_t magic;
     int n;
@@ -55,6 +60,8 @@ struct HIP_MAT_ {
     int *rowidx;      // device, nnz (row of each stored entry; built on demand)
     double *val64;    // device, nnz
     float *val32;     // device, nnz (built on demand)
+    int nlong;        // rows with more than kLongRow stored entries
+    int *long_rows;   // device, nlong: their indices, ascending (null when none)
     double *dinv64;   // device, n: 1 / a_ii (1 where a_ii is 0 or absent; built on demand)
     float *dinv32;    // device, n: the same, rounded to FP32
     int prec;         // precision of multivectors created from this matrix

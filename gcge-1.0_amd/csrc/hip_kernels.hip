@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <climits>
 #include <vector>
 
 #include "hip_common.h"
@@ -197,7 +198,8 @@ __device__ __forceinline__ double wave_colsum(double (&d)[KC], int lane)
 // before the first product is added, so a row costs one memory round trip
 // per batch rather than one per nonzero.  With DOT, the workgroup also
 // writes partial[j * nblk + blk] = sum_r X(r,j) Y(r,j) (wavefront sums by
-// wave_colsum, combined in LDS in a fixed order).
+// wave_colsum, combined in LDS in a fixed order) at partial[j * pstride +
+// blk].  Rows with more than long_thr entries are left to k_spmm_long.
 // STREAM (used for FP32): row blocks are renumbered so that each XCD walks a
 // contiguous range of rows (the plane neighbours of a stencil stay in its
 // L2), and the matrix and Y, each touched once, bypass the caches
@@ -210,7 +212,8 @@ __global__ __launch_bounds__(kSpmmRows) void k_spmm(int n, const int *__restrict
                                                     const T *__restrict__ val,
                                                     const T *__restrict__ X, int64_t ldx,
                                                     T *__restrict__ Y, int64_t ldy, int k,
-                                                    ColScalars g, double *__restrict__ partial, int nblk)
+                                                    ColScalars g, double *__restrict__ partial, int nblk,
+                                                    int long_thr, int pstride)
 {
     __shared__ int s_col[kSpmmCap];
     __shared__ T s_val[kSpmmCap];
@@ -239,14 +242,15 @@ __global__ __launch_bounds__(kSpmmRows) void k_spmm(int n, const int *__restrict
         }
     }
     __syncthreads();
-    const bool active = r < n;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     int s = 0, t = 0;
-    if (active) {
+    if (r < n) {
         s = rowptr[r];
         t = rowptr[r + 1];
     }
-    const int rs = active ? r : 0;  // a row index that is safe to gather
+    const bool active = r < n && t - s <= long_thr;  // long rows: k_spmm_long
+    if (!active) t = s;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int rs = r < n ? r : 0;  // a row index that is safe to gather
     for (int c0 = 0; c0 < k; c0 += KC) {
         const T *xb[KC];
 #pragma unroll
@@ -312,7 +316,56 @@ __global__ __launch_bounds__(kSpmmRows) void k_spmm(int n, const int *__restrict
     if (DOT) {
         __syncthreads();
         for (int c = threadIdx.x; c < k; c += kSpmmRows)
-            partial[(int64_t)c * nblk + blk] = (sdot[c][0] + sdot[c][1]) + (sdot[c][2] + sdot[c][3]);
+            partial[(int64_t)c * pstride + blk] = (sdot[c][0] + sdot[c][1]) + (sdot[c][2] + sdot[c][3]);
+    }
+}
+
+// The rows longer than kLongRow: one wavefront per row (four per workgroup),
+// lane l taking column c0 + l, so every column's sum runs over the row in CSR
+// order from 0, as in k_spmm (bit-exact); the row's entries go in batches of
+// kLongBatch whose column indices and values are read once for the
+// wavefront and whose X gathers are all issued before the first product is
+// added.  With DOT, the row's X(r,j) Y(r,j) goes to partial[j * pstride +
+// pbase + li].
+constexpr int kLongBatch = 8;
+
+template <typename T, bool DOT>
+__global__ __launch_bounds__(kSpmmRows) void k_spmm_long(const int *__restrict__ rowptr, const int *__restrict__ col,
+                                                         const T *__restrict__ val, const T *__restrict__ X,
+                                                         int64_t ldx, T *__restrict__ Y, int64_t ldy, int k,
+                                                         ColScalars g, const int *__restrict__ long_rows, int nlong,
+                                                         double *__restrict__ partial, int pstride, int pbase)
+{
+    const int lane = threadIdx.x & 63;
+    const int li = blockIdx.x * (kSpmmRows / 64) + (threadIdx.x >> 6);
+    if (li >= nlong) return;  // uniform per wavefront; no workgroup barrier here
+    const int r = long_rows[li];
+    const int s = rowptr[r], t = rowptr[r + 1];
+    for (int c0 = 0; c0 < k; c0 += 64) {
+        const int c = c0 + lane;
+        const bool live = c < k;
+        const int cc = live ? c : k - 1;
+        const T *xc = X + (g.use_idx ? (int64_t)g.idx[cc] : (int64_t)cc) * ldx;
+        T acc = T(0);
+        for (int e = s; e < t; e += kLongBatch) {
+            const int cnt = t - e;
+            int cj[kLongBatch];
+            T v[kLongBatch], xv[kLongBatch];
+#pragma unroll
+            for (int u = 0; u < kLongBatch; u++) {
+                cj[u] = u < cnt ? col[e + u] : r;
+                v[u] = u < cnt ? val[e + u] : T(0);
+            }
+#pragma unroll
+            for (int u = 0; u < kLongBatch; u++) xv[u] = xc[cj[u]];
+#pragma unroll
+            for (int u = 0; u < kLongBatch; u++)
+                if (u < cnt) acc = acc + v[u] * xv[u];
+        }
+        if (live) {
+            Y[(int64_t)c * ldy + r] = acc;
+            if (DOT) partial[(int64_t)c * pstride + pbase + li] = (double)xc[r] * (double)acc;
+        }
     }
 }
 
@@ -725,9 +778,13 @@ static bool spmm_t(HIP_MAT *A, const void *X, int64_t ldx, void *Y, int64_t ldy,
         ColScalars g = make_scalars(kc, nullptr, nullptr, xidx ? xidx + c0 : nullptr);
         const T *Xc = (const T *)X + (xidx ? 0 : (int64_t)c0 * ldx);
         T *Yc = (T *)Y + (int64_t)c0 * ldy;
+        // long rows (power-law graphs) go to k_spmm_long; their dot partials
+        // follow the workgroups' in each column
+        const int nlong = A->nlong, long_thr = nlong > 0 ? kLongRow : INT_MAX;
+        const int pstride = nblk + (DOT ? nlong : 0);
         double *partial = nullptr;
         if (DOT) {
-            partial = (double *)scratch(sizeof(double) * ((size_t)kc * nblk + kc));
+            partial = (double *)scratch(sizeof(double) * ((size_t)kc * pstride + kc));
             if (!partial) return false;
         }
         {
@@ -736,17 +793,21 @@ static bool spmm_t(HIP_MAT *A, const void *X, int64_t ldx, void *Y, int64_t ldy,
             ProfScope ps(DOT ? (f64 ? K_SPMM_DOT_F64 : K_SPMM_DOT_F32) : (f64 ? K_SPMM_F64 : K_SPMM_F32), bytes,
                          2.0 * kc * (double)A->nnz);
             if (f64)
-                k_spmm<T, DOT, 4, false><<<nblk, kSpmmRows, 0, ctx().stream>>>(A->n, A->rowptr, A->col, val, Xc, ldx,
-                                                                              Yc, ldy, kc, g, partial, nblk);
+                k_spmm<T, DOT, 4, false><<<nblk, kSpmmRows, 0, ctx().stream>>>(
+                    A->n, A->rowptr, A->col, val, Xc, ldx, Yc, ldy, kc, g, partial, nblk, long_thr, pstride);
             else if (DOT)
                 k_spmm<T, DOT, 4, true><<<((nblk + 7) / 8) * 8, kSpmmRows, 0, ctx().stream>>>(
-                    A->n, A->rowptr, A->col, val, Xc, ldx, Yc, ldy, kc, g, partial, nblk);
+                    A->n, A->rowptr, A->col, val, Xc, ldx, Yc, ldy, kc, g, partial, nblk, long_thr, pstride);
             else
                 k_spmm<T, DOT, 8, true><<<((nblk + 7) / 8) * 8, kSpmmRows, 0, ctx().stream>>>(
-                    A->n, A->rowptr, A->col, val, Xc, ldx, Yc, ldy, kc, g, partial, nblk);
+                    A->n, A->rowptr, A->col, val, Xc, ldx, Yc, ldy, kc, g, partial, nblk, long_thr, pstride);
+            if (nlong > 0)
+                k_spmm_long<T, DOT><<<(nlong + kSpmmRows / 64 - 1) / (kSpmmRows / 64), kSpmmRows, 0,
+                                      ctx().stream>>>(A->rowptr, A->col, val, Xc, ldx, Yc, ldy, kc, g, A->long_rows,
+                                                      nlong, partial, pstride, nblk);
         }
         if (!check(hipGetLastError(), "k_spmm")) return false;
-        if (DOT && !reduce_to_host(partial, kc, nblk, out + c0)) return false;
+        if (DOT && !reduce_to_host(partial, kc, pstride, out + c0)) return false;
     }
     return true;
 }
