@@ -6,8 +6,11 @@
 // dsyevx with range "I", dgemm, dsymm and BLAS-1.
 #include <dlfcn.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
+#include <vector>
 
 #include "gcge.h"
 
@@ -29,6 +32,7 @@ typedef void (*axpy_t)(int *, double *, double *, int *, double *, int *);
 typedef void (*copy_t)(int *, double *, int *, double *, int *);
 typedef void (*potrf_t)(const char *, int *, double *, int *, int *);
 typedef void (*trtri_t)(const char *, const char *, int *, double *, int *, int *);
+typedef void (*syev_t)(const char *, const char *, int *, double *, int *, double *, double *, int *, int *);
 
 struct Lapack {
     bool ok = false;
@@ -42,6 +46,7 @@ struct Lapack {
     copy_t dcopy_ = nullptr;
     potrf_t dpotrf_ = nullptr;  // used by the CholQR orthonormalization only
     trtri_t dtrtri_ = nullptr;
+    syev_t dsyev_ = nullptr;    // fallback of the subspace eigensolver (see d_eig)
 };
 
 Lapack &lapack()
@@ -67,6 +72,7 @@ Lapack &lapack()
     L.dcopy_ = (copy_t)dlsym(h, "dcopy_");
     L.dpotrf_ = (potrf_t)dlsym(h, "dpotrf_");
     L.dtrtri_ = (trtri_t)dlsym(h, "dtrtri_");
+    L.dsyev_ = (syev_t)dlsym(h, "dsyev_");
     L.ok = L.dsyevx_ && L.dgemm_ && L.dsymm_ && L.ddot_ && L.dnrm2_ && L.dscal_ && L.daxpy_ && L.dcopy_;
     if (!L.ok) fprintf(stderr, "GCGE: LAPACK library %s lacks a required routine\n", name);
     return L;
@@ -100,13 +106,42 @@ double lapack_nrm2(int *n, double *x, int *incx)
 
 namespace {
 
+// dsyevx as the reference calls it (gcge_ops.c:302-336).  When dsyevx
+// reports eigenvectors that failed to converge (info > 0: inverse iteration
+// inside a cluster of equal eigenvalues, as when a graph has more isolated
+// vertices than eigenpairs are wanted), the pairs il..iu are recomputed from
+// the full decomposition of dsyev (QR iteration, orthonormal eigenvectors for
+// any spectrum) of the same matrix; info then reports dsyev's outcome.
 void d_eig(char *jobz, char *range, char *uplo, GCGE_INT *nrows, GCGE_DOUBLE *a, GCGE_INT *lda, GCGE_DOUBLE *vl,
            GCGE_DOUBLE *vu, GCGE_INT *il, GCGE_INT *iu, GCGE_DOUBLE *abstol, GCGE_INT *nev, GCGE_DOUBLE *eval,
            GCGE_DOUBLE *evec, GCGE_INT *lde, GCGE_INT *, GCGE_DOUBLE *work, GCGE_INT *lwork, GCGE_INT *iwork,
            GCGE_INT *, GCGE_INT *ifail, GCGE_INT *info)
 {
+    Lapack &L = lapack();
+    const int n = *nrows, ld = *lda;
+    const bool can_redo = L.dsyev_ && *jobz == 'V' && *range == 'I' && n > 0;
+    std::vector<double> keep;
+    // dsyevx destroys its triangle of a; the last column ends at row n
+    if (can_redo) keep.assign(a, a + (size_t)ld * (n - 1) + n);
     LAPACK_CALL(dsyevx_, jobz, range, uplo, nrows, a, lda, vl, vu, il, iu, abstol, nev, eval, evec, lde, work, lwork,
                 iwork, ifail, info);
+    if (!can_redo || *info <= 0) return;
+    std::vector<double> w(n), wk(std::max(1, 3 * n - 1));
+    int lw = (int)wk.size(), inf = 0, nn = n, ldk = ld;
+    keep.resize((size_t)ld * n);
+    L.dsyev_("V", uplo, &nn, keep.data(), &ldk, w.data(), wk.data(), &lw, &inf);
+    if (inf != 0) {
+        *info = inf;
+        return;
+    }
+    const int m = *iu - *il + 1;
+    for (int j = 0; j < m; j++) {
+        eval[j] = w[*il - 1 + j];
+        std::memcpy(evec + (size_t)j * *lde, keep.data() + (size_t)(*il - 1 + j) * ld, sizeof(double) * n);
+        ifail[j] = 0;
+    }
+    *nev = m;
+    *info = 0;
 }
 
 void d_gemm(char *ta, char *tb, GCGE_INT *m, GCGE_INT *n, GCGE_INT *k, GCGE_DOUBLE *alpha, GCGE_DOUBLE *a,

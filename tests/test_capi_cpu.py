@@ -141,3 +141,50 @@ def test_no_gpu_calls_fail_cleanly_without_device():
     assert code != 0 and msg
     L.GCGE_HIP_ClearError()
     assert capi.last_error()[0] == 0
+
+
+def _dense_eig_slot():
+    L = capi.load()
+    ops = capi.GCGE_OPS()
+    L.GCGE_SetDenseOps.argtypes = [C.POINTER(capi.GCGE_OPS)]
+    L.GCGE_SetDenseOps.restype = None
+    L.GCGE_SetDenseOps(C.byref(ops))
+    I, D = C.POINTER(C.c_int), C.POINTER(C.c_double)
+    proto = C.CFUNCTYPE(None, C.c_char_p, C.c_char_p, C.c_char_p, I, D, I, D, D, I, I, D, I, D, D, I, I, D, I, I,
+                        I, I, I)
+    return proto(ops.DenseMatEigenSolver)
+
+
+@pytest.mark.parametrize("cluster", [0, 40, 150])
+def test_dense_eigensolver_range_on_degenerate_clusters(cluster):
+    # the subspace eigenproblem of the outer iteration (src/gcge_ops.c:302-336,
+    # dsyevx range "I"): pairs il..iu of a symmetric matrix whose lowest
+    # `cluster` eigenvalues are equal (a graph Laplacian plus a shift with
+    # many isolated vertices gives such Ritz matrices); the pairs must be
+    # the lowest eigenvalues and an orthonormal set of eigenvectors
+    eig = _dense_eig_slot()
+    n, il, iu = 208, 1, 108
+    rng = np.random.default_rng(cluster + 7)
+    lam = np.sort(np.concatenate([np.full(cluster, 1e-2), 1e-2 + rng.uniform(1e-3, 5.0, n - cluster)]))
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    A = (Q * lam) @ Q.T
+    A = 0.5 * (A + A.T)
+    a = np.asfortranarray(A.copy())
+    m = iu - il + 1
+    ev = np.zeros(n)  # dsyevx may use all n entries
+    V = np.zeros((n, m), order="F")
+    ci = lambda v: C.byref(C.c_int(v))
+    nev, info = C.c_int(0), C.c_int(-99)
+    lwork = 8 * n
+    work, iwork, ifail = np.zeros(lwork), np.zeros(5 * n, dtype=np.int32), np.zeros(n, dtype=np.int32)
+    isuppz = np.zeros(2 * n, dtype=np.int32)
+    dp = lambda x: x.ctypes.data_as(C.POINTER(C.c_double))
+    ip = lambda x: x.ctypes.data_as(C.POINTER(C.c_int))
+    eig(b"V", b"I", b"U", ci(n), dp(a), ci(n), C.byref(C.c_double(0)), C.byref(C.c_double(0)), ci(il), ci(iu),
+        C.byref(C.c_double(0)), C.byref(nev), dp(ev), dp(V), ci(n), ip(isuppz), dp(work), ci(lwork), ip(iwork),
+        ci(5 * n), ip(ifail), C.byref(info))
+    assert info.value == 0 and nev.value == m
+    ev = ev[:m]
+    np.testing.assert_allclose(ev, lam[il - 1:iu], rtol=1e-12, atol=1e-13)
+    assert np.abs(V.T @ V - np.eye(m)).max() < 1e-12
+    assert np.abs(A @ V - V * ev).max() < 1e-12 * np.abs(lam).max() * 10
