@@ -93,6 +93,24 @@ void *scratch(size_t bytes)
     return g_ctx.scratch;
 }
 
+void *row_stage(size_t bytes)
+{
+    if (!ensure_ctx()) return nullptr;
+    if (bytes <= g_ctx.rows_bytes) return g_ctx.rows;
+    if (g_ctx.rows) {
+        if (!sync()) return nullptr;  // kernels queued on the stream may still read it
+        (void)hipFree(g_ctx.rows);
+        g_ctx.dev_bytes -= (double)g_ctx.rows_bytes;
+        g_ctx.rows = nullptr;
+        g_ctx.rows_bytes = 0;
+    }
+    const size_t want = (bytes + ((size_t)1 << 20) - 1) & ~(((size_t)1 << 20) - 1);
+    if (!check(hipMalloc(&g_ctx.rows, want), "hipMalloc row staging")) return nullptr;
+    g_ctx.rows_bytes = want;
+    g_ctx.dev_bytes += (double)want;
+    return g_ctx.rows;
+}
+
 double *host_buf(size_t count)
 {
     if (!ensure_ctx()) return nullptr;
@@ -373,6 +391,8 @@ static bool mat_ok(HIP_MAT *A, MV *x, const char *what)
 using namespace gh;
 
 // ====================================================================== matrices
+static int pick_gather_rows(int n, const int *rowptr, const int *col);
+
 extern "C" HIP_MAT *HIP_MatCreateFromCSR(GCGE_INT nrows, GCGE_INT ncols, GCGE_INT nnz, const GCGE_INT *rowptr,
                                          const GCGE_INT *col, const GCGE_DOUBLE *val)
 {
@@ -416,6 +436,7 @@ extern "C" HIP_MAT *HIP_MatCreateFromCSR(GCGE_INT nrows, GCGE_INT ncols, GCGE_IN
     for (int i = 0; i < nrows; i++)
         if (rowptr[i + 1] - rowptr[i] > kLongRow) longs.push_back(i);
     A->nlong = (int)longs.size();
+    A->gather_rows = pick_gather_rows(nrows, rowptr, col);
     bool ok = check(hipMalloc((void **)&A->rowptr, sizeof(int) * (nrows + 1)), "hipMalloc rowptr") &&
               check(hipMalloc((void **)&A->col, sizeof(int) * std::max(1, nnz)), "hipMalloc col") &&
               check(hipMalloc((void **)&A->val64, sizeof(double) * std::max(1, nnz)), "hipMalloc val");
@@ -434,6 +455,33 @@ extern "C" HIP_MAT *HIP_MatCreateFromCSR(GCGE_INT nrows, GCGE_INT ncols, GCGE_IN
     }
     g_ctx.dev_bytes += sizeof(int) * (double)(nrows + 1) + 12.0 * nnz;
     return A;
+}
+
+// Gather layout of the SpMM.  A wavefront of the column-major kernel reads,
+// per stored entry and column, the cache lines that its 64 rows' column
+// indices fall in; a stencil's rows share a few lines (about 0.03 distinct
+// 32-column segments per entry over a 64-row block), a random graph's do not
+// (0.4 for a Chung-Lu graph).  Above 0.15 the row-major path is used.
+// GCGE_HIP_SPMM_GATHER=rows|columns overrides the choice for matrices
+// created afterwards.
+static int pick_gather_rows(int n, const int *rowptr, const int *col)
+{
+    const char *e = getenv("GCGE_HIP_SPMM_GATHER");
+    if (e && !strcmp(e, "rows")) return 1;
+    if (e && !strcmp(e, "columns")) return 0;
+    const int nb = n / 64;
+    if (nb == 0) return 0;
+    const int step = std::max(1, nb / 2048);
+    std::vector<int> seg;
+    double lines = 0.0, ents = 0.0;
+    for (int b = 0; b < nb; b += step) {
+        seg.assign(col + rowptr[b * 64], col + rowptr[b * 64 + 64]);
+        for (int &x : seg) x >>= 5;
+        std::sort(seg.begin(), seg.end());
+        lines += (double)(std::unique(seg.begin(), seg.end()) - seg.begin());
+        ents += (double)seg.size();
+    }
+    return ents > 0.0 && lines > 0.15 * ents ? 1 : 0;
 }
 
 extern "C" HIP_MAT *HIP_ReadMatFile(const char *filename)

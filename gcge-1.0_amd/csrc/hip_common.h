@@ -66,6 +66,7 @@ struct HIP_MAT_ {
     float *dinv32;    // device, n: the same, rounded to FP32
     int prec;         // precision of multivectors created from this matrix
     int max_row_nnz;
+    int gather_rows;  // SpMM gathers from a row-major copy of X (scattered column indices)
     double row_abs_max;  // max_i sum_j |a_ij| (Gershgorin bound of ||A||_2)
 };
 
@@ -78,6 +79,8 @@ struct Ctx {
     // device scratch for partial reductions and staging (grown on demand)
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
+    void *rows = nullptr;  // row-major X staging of the SpMM gather path
+    size_t rows_bytes = 0;
     // pinned host buffer for results read back by the host
     double *hbuf = nullptr;
     size_t hbuf_bytes = 0;
@@ -93,6 +96,7 @@ struct Ctx {
 Ctx &ctx();
 bool ensure_ctx();
 void *scratch(size_t bytes);           // device scratch of at least bytes
+void *row_stage(size_t bytes);         // device staging of row-major X copies (SpMM), at least bytes
 double *host_buf(size_t count);        // pinned host doubles
 // Copy small host data (coefficients) to device memory without blocking:
 // the bytes go through a pinned ring whose slots are reused only after the

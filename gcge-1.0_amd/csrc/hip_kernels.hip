@@ -369,6 +369,173 @@ __global__ __launch_bounds__(kSpmmRows) void k_spmm_long(const int *__restrict__
     }
 }
 
+// ------------------------------------------------- SpMM, row-major gather
+// Matrices whose column indices scatter (random graphs) make the column-major
+// gather of k_spmm touch one cache line per stored entry and column.  For
+// them (HIP_MAT::gather_rows) the k columns are first copied into a
+// row-major staging block Xt, row r holding X(r, 0..kp) (kp: k rounded up to
+// the lane group G), and k_spmm_rows gathers whole row segments: G lanes of a
+// wavefront take G consecutive columns of one row, so a stored entry costs
+// one contiguous G * sizeof(T) read.  Every sum still runs over its row in
+// CSR order from 0, so Y is bit-identical to the column-major path.
+
+// Xt(r, c) = X(:, c)(r) (c < k; 0 for k <= c < kp): 64 x 64 tiles through LDS.
+template <typename T>
+__global__ __launch_bounds__(256) void k_rows_from_cols(int n, const T *__restrict__ X, int64_t ldx, int k,
+                                                        ColScalars g, int kp, T *__restrict__ Xt)
+{
+    __shared__ T tile[64][65];
+    const int r0 = blockIdx.x * 64, cb = blockIdx.y * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int cc = ty; cc < 64; cc += 4) {
+        const int c = cb + cc, r = r0 + tx;
+        T v = T(0);
+        if (c < k && r < n) v = X[(g.use_idx ? (int64_t)g.idx[c] : (int64_t)c) * ldx + r];
+        tile[cc][tx] = v;
+    }
+    __syncthreads();
+    const int kpe = min(64, kp - cb), rows = min(64, n - r0);
+    for (int q = threadIdx.x; q < rows * kpe; q += 256) {
+        const int rr = q / kpe, cc = q - rr * kpe;
+        Xt[(int64_t)(r0 + rr) * kp + cb + cc] = tile[cc][rr];
+    }
+}
+
+__device__ __forceinline__ float lane_bcast(float v, int src)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+
+__device__ __forceinline__ double lane_bcast(double v, int src)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, src);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+constexpr int kRowsBatch = 8;       // gathers in flight per lane, short rows
+constexpr int kLongRowsBatch = 32;  // gathers in flight per lane, long rows
+
+// Rows handled by one workgroup of k_spmm_rows<G>: four wavefronts of
+// max(16, 64 / G) rows each.
+__host__ __device__ constexpr int rows_per_wg(int G) { return 4 * (64 / G > 16 ? 64 / G : 16); }
+
+// Y(:, c0 + 0..G) = A X from the staging block Xt (grid.y: chunks of 64
+// columns when G = 64).  Workgroups below nlb take the rows longer than
+// long_thr, one wavefront per row with lane l on column c0 + l; the row's
+// column indices and values are read 64 at a time and broadcast from the
+// lane holding them (readlane), so the gathers have uniform row addresses
+// and kLongRowsBatch of them are in flight.  The other workgroups take
+// rows_per_wg(G) consecutive rows: lane l works on column c0 + l % G of row
+// l / G (64 / G rows in flight per wavefront), the sums land in LDS and
+// leave as contiguous column segments.  With DOT, sum_r X(r,c) Y(r,c) goes
+// to partial[c * pstride + blk] per row workgroup and to
+// partial[c * pstride + nblk + li] per long row.
+template <typename T, bool DOT, int G>
+__global__ __launch_bounds__(256) void k_spmm_rows(int n, const int *__restrict__ rowptr,
+                                                   const int *__restrict__ col, const T *__restrict__ val,
+                                                   const T *__restrict__ Xt, int kp, T *__restrict__ Y,
+                                                   int64_t ldy, int k, const int *__restrict__ long_rows,
+                                                   int nlong, int nlb, int long_thr,
+                                                   double *__restrict__ partial, int pstride, int nblk)
+{
+    static_assert(G >= 2 && G <= 64 && (G & (G - 1)) == 0, "G is a power of two");
+    constexpr int SG = 64 / G;
+    constexpr int RWG = rows_per_wg(G);
+    constexpr int RPW = RWG / 4;
+    __shared__ T sY[G][RWG + 1];
+    __shared__ unsigned char skip[RWG];
+    __shared__ double sdot[DOT ? 4 : 1][G];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int c0 = blockIdx.y * 64;
+    if ((int)blockIdx.x < nlb) {
+        const int li = blockIdx.x * 4 + wid;
+        if (li >= nlong) return;  // uniform per wavefront; this branch has no barrier
+        const int r = long_rows[li];
+        const int s = rowptr[r], t = rowptr[r + 1];
+        const int c = c0 + lane;
+        const bool live = c < k;
+        const int cc = min(c, kp - 1);
+        T acc = T(0);
+        for (int e = s; e < t; e += 64) {
+            const int cnt = min(64, t - e);
+            const int cv = lane < cnt ? col[e + lane] : r;
+            const T vv = lane < cnt ? val[e + lane] : T(0);
+#pragma unroll
+            for (int h = 0; h < 64; h += kLongRowsBatch) {
+                if (h < cnt) {
+                    T xv[kLongRowsBatch];
+#pragma unroll
+                    for (int u = 0; u < kLongRowsBatch; u++) {
+                        const int cj = __builtin_amdgcn_readlane(cv, h + u);
+                        xv[u] = Xt[(int64_t)cj * kp + cc];
+                    }
+#pragma unroll
+                    for (int u = 0; u < kLongRowsBatch; u++)
+                        if (h + u < cnt) acc = acc + lane_bcast(vv, h + u) * xv[u];
+                }
+            }
+        }
+        if (live) {
+            Y[(int64_t)c * ldy + r] = acc;
+            if (DOT) partial[(int64_t)c * pstride + nblk + li] = (double)Xt[(int64_t)r * kp + cc] * (double)acc;
+        }
+        return;
+    }
+    const int blk = blockIdx.x - nlb;
+    const int r0 = blk * RWG;
+    const int cl = lane % G, sg = lane / G;
+    const int c = c0 + cl;  // < kp: kp = G, or a multiple of 64 when G = 64
+    const bool live = c < k;
+    double dsum = 0.0;
+    for (int i = sg; i < RPW; i += SG) {
+        const int rl = wid * RPW + i, r = r0 + rl;
+        int s = 0, t = 0;
+        if (r < n) {
+            s = rowptr[r];
+            t = rowptr[r + 1];
+        }
+        const bool act = r < n && t - s <= long_thr;
+        if (!act) t = s;
+        const int rs = r < n ? r : 0;
+        T acc = T(0);
+        for (int e = s; e < t; e += kRowsBatch) {
+            const int cnt = t - e;
+            int cj[kRowsBatch];
+            T v[kRowsBatch], xv[kRowsBatch];
+#pragma unroll
+            for (int u = 0; u < kRowsBatch; u++) {
+                cj[u] = u < cnt ? col[e + u] : rs;
+                v[u] = u < cnt ? val[e + u] : T(0);
+            }
+#pragma unroll
+            for (int u = 0; u < kRowsBatch; u++) xv[u] = Xt[(int64_t)cj[u] * kp + c];
+#pragma unroll
+            for (int u = 0; u < kRowsBatch; u++)
+                if (u < cnt) acc = acc + v[u] * xv[u];
+        }
+        if (cl == 0) skip[rl] = act ? 0 : 1;
+        sY[cl][rl] = acc;
+        if (DOT && act && live) dsum += (double)Xt[(int64_t)r * kp + c] * (double)acc;
+    }
+    if (DOT) {
+#pragma unroll
+        for (int m = G; m < 64; m <<= 1) dsum += __shfl_xor(dsum, m, 64);
+        if (sg == 0) sdot[wid][cl] = dsum;
+    }
+    __syncthreads();
+    const int ncol = min(G, k - c0);
+    for (int q = threadIdx.x; q < ncol * RWG; q += 256) {
+        const int cc = q / RWG, rr = q - cc * RWG;
+        const int r = r0 + rr;
+        if (r < n && !skip[rr]) Y[(int64_t)(c0 + cc) * ldy + r] = sY[cc][rr];
+    }
+    if (DOT && (int)threadIdx.x < ncol)
+        partial[(int64_t)(c0 + threadIdx.x) * pstride + blk] =
+            (sdot[0][threadIdx.x] + sdot[1][threadIdx.x]) + (sdot[2][threadIdx.x] + sdot[3][threadIdx.x]);
+}
+
 // ---------------------------------------------------------------- residual
 // r = A x - lambda B x (x when B is absent), in FP64 arithmetic whatever the
 // storage precision of x, rounded as VecAxpby(-lambda, Bx, 1.0, Ax)
@@ -469,6 +636,123 @@ __global__ __launch_bounds__(kSpmmRows) void k_residual(int n, const int *__rest
     const int nb = gridDim.x;
     for (int q = threadIdx.x; q < 3 * k; q += kSpmmRows)
         partial[(int64_t)q * nb + blk] = (sred[q][0] + sred[q][1]) + (sred[q][2] + sred[q][3]);
+}
+
+// r = A x - lambda x (B = I) on the row-major gather path: the FP64 row sums
+// of k_residual (CSR order from 0, x converted to FP64 per entry) gathered
+// from the staged rows Xt, in the workgroup layout of k_spmm_rows (long rows
+// one wavefront each).  The sums r.r, x.x and Bx.Bx (= x.x) of column c go
+// to partial[(3c + q) * pstride + blk] per row workgroup and to
+// partial[(3c + q) * pstride + nblk + li] per long row.
+template <typename TX, int G>
+__global__ __launch_bounds__(256) void k_residual_rows(int n, const int *__restrict__ rowptr,
+                                                       const int *__restrict__ col, const double *__restrict__ val,
+                                                       const TX *__restrict__ Xt, int kp, int k, ColScalars g,
+                                                       const int *__restrict__ long_rows, int nlong, int nlb,
+                                                       int long_thr, double *__restrict__ partial, int pstride,
+                                                       int nblk)
+{
+    constexpr int SG = 64 / G;
+    constexpr int RWG = rows_per_wg(G);
+    constexpr int RPW = RWG / 4;
+    __shared__ double sred[4][G][2];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int c0 = blockIdx.y * 64;
+    if ((int)blockIdx.x < nlb) {
+        const int li = blockIdx.x * 4 + wid;
+        if (li >= nlong) return;  // uniform per wavefront; this branch has no barrier
+        const int r = long_rows[li];
+        const int s = rowptr[r], t = rowptr[r + 1];
+        const int c = c0 + lane;
+        const int cc = min(c, kp - 1);
+        double ax = 0.0;
+        for (int e = s; e < t; e += 64) {
+            const int cnt = min(64, t - e);
+            const int cv = lane < cnt ? col[e + lane] : r;
+            const double vv = lane < cnt ? val[e + lane] : 0.0;
+#pragma unroll
+            for (int h = 0; h < 64; h += kLongRowsBatch) {
+                if (h < cnt) {
+                    TX xv[kLongRowsBatch];
+#pragma unroll
+                    for (int u = 0; u < kLongRowsBatch; u++) {
+                        const int cj = __builtin_amdgcn_readlane(cv, h + u);
+                        xv[u] = Xt[(int64_t)cj * kp + cc];
+                    }
+#pragma unroll
+                    for (int u = 0; u < kLongRowsBatch; u++)
+                        if (h + u < cnt) ax = ax + lane_bcast(vv, h + u) * (double)xv[u];
+                }
+            }
+        }
+        if (c < k) {
+            const double xr = (double)Xt[(int64_t)r * kp + cc];
+            const double res = axpby_el<double>(-g.a[c], xr, 1.0, ax);
+            partial[(int64_t)(3 * c + 0) * pstride + nblk + li] = res * res;
+            partial[(int64_t)(3 * c + 1) * pstride + nblk + li] = xr * xr;
+            partial[(int64_t)(3 * c + 2) * pstride + nblk + li] = xr * xr;
+        }
+        return;
+    }
+    const int blk = blockIdx.x - nlb;
+    const int r0 = blk * RWG;
+    const int cl = lane % G, sg = lane / G;
+    const int c = c0 + cl;  // < kp
+    const bool live = c < k;
+    double srr = 0.0, sxx = 0.0;
+    for (int i = sg; i < RPW; i += SG) {
+        const int r = r0 + wid * RPW + i;
+        int s = 0, t = 0;
+        if (r < n) {
+            s = rowptr[r];
+            t = rowptr[r + 1];
+        }
+        const bool act = r < n && t - s <= long_thr;
+        if (!act) t = s;
+        const int rs = r < n ? r : 0;
+        double ax = 0.0;
+        for (int e = s; e < t; e += kRowsBatch) {
+            const int cnt = t - e;
+            int cj[kRowsBatch];
+            double v[kRowsBatch];
+            TX xv[kRowsBatch];
+#pragma unroll
+            for (int u = 0; u < kRowsBatch; u++) {
+                cj[u] = u < cnt ? col[e + u] : rs;
+                v[u] = u < cnt ? val[e + u] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < kRowsBatch; u++) xv[u] = Xt[(int64_t)cj[u] * kp + c];
+#pragma unroll
+            for (int u = 0; u < kRowsBatch; u++)
+                if (u < cnt) ax = ax + v[u] * (double)xv[u];
+        }
+        if (act && live) {
+            const double xr = (double)Xt[(int64_t)r * kp + c];
+            const double res = axpby_el<double>(-g.a[c], xr, 1.0, ax);
+            srr += res * res;
+            sxx += xr * xr;
+        }
+    }
+#pragma unroll
+    for (int m = G; m < 64; m <<= 1) {
+        srr += __shfl_xor(srr, m, 64);
+        sxx += __shfl_xor(sxx, m, 64);
+    }
+    if (sg == 0) {
+        sred[wid][cl][0] = srr;
+        sred[wid][cl][1] = sxx;
+    }
+    __syncthreads();
+    const int q = threadIdx.x;
+    if (q < min(G, k - c0)) {
+        const int cq = c0 + q;
+        const double rr = (sred[0][q][0] + sred[1][q][0]) + (sred[2][q][0] + sred[3][q][0]);
+        const double xx = (sred[0][q][1] + sred[1][q][1]) + (sred[2][q][1] + sred[3][q][1]);
+        partial[(int64_t)(3 * cq + 0) * pstride + blk] = rr;
+        partial[(int64_t)(3 * cq + 1) * pstride + blk] = xx;
+        partial[(int64_t)(3 * cq + 2) * pstride + blk] = xx;
+    }
 }
 
 // ---------------------------------------------------------------- CG update
@@ -766,6 +1050,49 @@ const double *mat_vals<double>(HIP_MAT *A) { return A->val64; }
 template <>
 const float *mat_vals<float>(HIP_MAT *A) { return mat_val32(A); }
 
+// Row-major gather path (see k_spmm_rows) for kc <= kMaxCols columns.
+template <typename T, bool DOT, int G>
+static bool spmm_rows_g(HIP_MAT *A, const T *val, const T *Xc, int64_t ldx, ColScalars g, T *Yc, int64_t ldy,
+                        int kc, double *out)
+{
+    const int n = A->n, kp = G < 64 ? G : ((kc + 63) / 64) * 64;
+    T *Xt = (T *)row_stage(sizeof(T) * (size_t)n * kp);
+    if (!Xt) return false;
+    const int RWG = rows_per_wg(G);
+    const int nblk = (n + RWG - 1) / RWG, nlong = A->nlong, nlb = (nlong + 3) / 4;
+    const int pstride = nblk + (DOT ? nlong : 0);
+    double *partial = nullptr;
+    if (DOT) {
+        partial = (double *)scratch(sizeof(double) * ((size_t)kc * pstride + kc));
+        if (!partial) return false;
+    }
+    {
+        const double bytes = (sizeof(T) + 4.0) * A->nnz + 4.0 * (n + 1) + 2.0 * kc * (double)n * sizeof(T);
+        const bool f64 = sizeof(T) == 8;
+        ProfScope ps(DOT ? (f64 ? K_SPMM_DOT_F64 : K_SPMM_DOT_F32) : (f64 ? K_SPMM_F64 : K_SPMM_F32), bytes,
+                     2.0 * kc * (double)A->nnz);
+        k_rows_from_cols<T><<<dim3((n + 63) / 64, (kp + 63) / 64), 256, 0, ctx().stream>>>(n, Xc, ldx, kc, g, kp,
+                                                                                            Xt);
+        k_spmm_rows<T, DOT, G><<<dim3(nlb + nblk, G < 64 ? 1 : kp / 64), 256, 0, ctx().stream>>>(
+            n, A->rowptr, A->col, val, Xt, kp, Yc, ldy, kc, A->long_rows, nlong, nlb,
+            nlong > 0 ? kLongRow : INT_MAX, partial, pstride, nblk);
+    }
+    if (!check(hipGetLastError(), "k_spmm_rows")) return false;
+    return !DOT || reduce_to_host(partial, kc, pstride, out);
+}
+
+template <typename T, bool DOT>
+static bool spmm_rows(HIP_MAT *A, const T *val, const T *Xc, int64_t ldx, ColScalars g, T *Yc, int64_t ldy,
+                      int kc, double *out)
+{
+    if (kc > 32) return spmm_rows_g<T, DOT, 64>(A, val, Xc, ldx, g, Yc, ldy, kc, out);
+    if (kc > 16) return spmm_rows_g<T, DOT, 32>(A, val, Xc, ldx, g, Yc, ldy, kc, out);
+    if (kc > 8) return spmm_rows_g<T, DOT, 16>(A, val, Xc, ldx, g, Yc, ldy, kc, out);
+    if (kc > 4) return spmm_rows_g<T, DOT, 8>(A, val, Xc, ldx, g, Yc, ldy, kc, out);
+    if (kc > 2) return spmm_rows_g<T, DOT, 4>(A, val, Xc, ldx, g, Yc, ldy, kc, out);
+    return spmm_rows_g<T, DOT, 2>(A, val, Xc, ldx, g, Yc, ldy, kc, out);
+}
+
 template <typename T, bool DOT>
 static bool spmm_t(HIP_MAT *A, const void *X, int64_t ldx, void *Y, int64_t ldy, int k, const int *xidx,
                    double *out)
@@ -778,6 +1105,10 @@ static bool spmm_t(HIP_MAT *A, const void *X, int64_t ldx, void *Y, int64_t ldy,
         ColScalars g = make_scalars(kc, nullptr, nullptr, xidx ? xidx + c0 : nullptr);
         const T *Xc = (const T *)X + (xidx ? 0 : (int64_t)c0 * ldx);
         T *Yc = (T *)Y + (int64_t)c0 * ldy;
+        if (A->gather_rows && kc >= 2) {  // one column gathers a line per entry either way
+            if (!spmm_rows<T, DOT>(A, val, Xc, ldx, g, Yc, ldy, kc, DOT ? out + c0 : nullptr)) return false;
+            continue;
+        }
         // long rows (power-law graphs) go to k_spmm_long; their dot partials
         // follow the workgroups' in each column
         const int nlong = A->nlong, long_thr = nlong > 0 ? kLongRow : INT_MAX;
@@ -827,6 +1158,43 @@ bool spmm_dot(int prec, HIP_MAT *A, const void *X, int64_t ldx, void *Y, int64_t
     return spmm_t<float, true>(A, X, ldx, Y, ldy, k, nullptr, out);
 }
 
+// Residual sums of kc <= kMaxCols columns on the row-major gather path (B = I);
+// out[3j + q]: r.r, x.x, x.x of column j.
+template <typename T, int G>
+static bool residual_rows_g(HIP_MAT *A, const T *X, int64_t ldx, ColScalars g, int kc, double *out)
+{
+    const int n = A->n, kp = G < 64 ? G : ((kc + 63) / 64) * 64;
+    T *Xt = (T *)row_stage(sizeof(T) * (size_t)n * kp);
+    if (!Xt) return false;
+    const int RWG = rows_per_wg(G);
+    const int nblk = (n + RWG - 1) / RWG, nlong = A->nlong, nlb = (nlong + 3) / 4;
+    const int pstride = nblk + nlong;
+    double *partial = (double *)scratch(sizeof(double) * ((size_t)3 * kc * pstride + 3 * kc));
+    if (!partial) return false;
+    {
+        ProfScope ps(K_RESIDUAL, 12.0 * A->nnz + 4.0 * (n + 1) + (double)kc * n * sizeof(T),
+                     2.0 * kc * (double)A->nnz);
+        k_rows_from_cols<T><<<dim3((n + 63) / 64, (kp + 63) / 64), 256, 0, ctx().stream>>>(n, X, ldx, kc, g, kp,
+                                                                                           Xt);
+        k_residual_rows<T, G><<<dim3(nlb + nblk, G < 64 ? 1 : kp / 64), 256, 0, ctx().stream>>>(
+            n, A->rowptr, A->col, A->val64, Xt, kp, kc, g, A->long_rows, nlong, nlb,
+            nlong > 0 ? kLongRow : INT_MAX, partial, pstride, nblk);
+    }
+    if (!check(hipGetLastError(), "k_residual_rows")) return false;
+    return reduce_to_host(partial, 3 * kc, pstride, out);
+}
+
+template <typename T>
+static bool residual_rows(HIP_MAT *A, const T *X, int64_t ldx, ColScalars g, int kc, double *out)
+{
+    if (kc > 32) return residual_rows_g<T, 64>(A, X, ldx, g, kc, out);
+    if (kc > 16) return residual_rows_g<T, 32>(A, X, ldx, g, kc, out);
+    if (kc > 8) return residual_rows_g<T, 16>(A, X, ldx, g, kc, out);
+    if (kc > 4) return residual_rows_g<T, 8>(A, X, ldx, g, kc, out);
+    if (kc > 2) return residual_rows_g<T, 4>(A, X, ldx, g, kc, out);
+    return residual_rows_g<T, 2>(A, X, ldx, g, kc, out);
+}
+
 template <typename T>
 static bool residual_t(HIP_MAT *A, HIP_MAT *B, const void *X, int64_t ldx, const int *idx, int k,
                        const double *lambda, double *rnorm, double *xnorm, double *bxnorm)
@@ -836,6 +1204,15 @@ static bool residual_t(HIP_MAT *A, HIP_MAT *B, const void *X, int64_t ldx, const
     for (int c0 = 0; c0 < k; c0 += kMaxCols) {
         const int kc = std::min(kMaxCols, k - c0);
         ColScalars g = make_scalars(kc, lambda + c0, nullptr, idx + c0);
+        if (!B && A->gather_rows && kc >= 2) {
+            if (!residual_rows<T>(A, (const T *)X, ldx, g, kc, tmp.data())) return false;
+            for (int j = 0; j < kc; j++) {
+                rnorm[c0 + j] = std::sqrt(tmp[3 * j + 0]);
+                xnorm[c0 + j] = std::sqrt(tmp[3 * j + 1]);
+                if (bxnorm) bxnorm[c0 + j] = std::sqrt(tmp[3 * j + 2]);
+            }
+            continue;
+        }
         double *partial = (double *)scratch(sizeof(double) * ((size_t)3 * kc * nblk + 3 * kc));
         if (!partial) return false;
         const double rbytes = 12.0 * A->nnz + 4.0 * (A->n + 1) + (B ? 12.0 * B->nnz + 4.0 * (B->n + 1) : 0.0) +

@@ -96,6 +96,66 @@ def test_spmm_column_offsets(ops):
     assert np.all(Y[:, :5] == 0) and np.all(Y[:, 11:] == 0)
 
 
+def _matrix_with_gather(A, layout, monkeypatch):
+    # the SpMM gather layout is chosen when the matrix is created
+    monkeypatch.setenv("GCGE_HIP_SPMM_GATHER", layout)
+    m = Matrix(A)
+    monkeypatch.delenv("GCGE_HIP_SPMM_GATHER")
+    return m
+
+
+@pytest.mark.parametrize("name", list(MATS))
+@pytest.mark.parametrize("k", [2, 3, 5, 8, 16, 17, 33, 64, 70, 128, 130])
+def test_spmm_row_gather_fp64_bit_exact(ops, monkeypatch, name, k):
+    # row-major gather path (staged X rows, lanes over columns, long rows one
+    # wavefront each): same CSR-order sums as the column path, so the same bits
+    A = MATS[name]
+    n = A.shape[0]
+    X = rng(k + 11).standard_normal((n, k))
+    mA = _matrix_with_gather(A, "rows", monkeypatch)
+    x = MultiVec(mA, k, data=X)
+    y = MultiVec(mA, k)
+    ops.matdot(mA, x, y)
+    Y = O.csr_matmultivec(A, X)
+    np.testing.assert_array_equal(y.get(), Y)
+    y2 = MultiVec(mA, k)
+    d = ops.matdot_dot(mA, x, y2)
+    np.testing.assert_array_equal(y2.get(), Y)
+    np.testing.assert_allclose(d, np.einsum("ij,ij->j", X, Y), rtol=RTOL_RED)
+
+
+@pytest.mark.parametrize("name", ["vardiff2d", "chunglu"])
+@pytest.mark.parametrize("k", [2, 6, 40, 100])
+def test_spmm_row_gather_fp32_matches_column_gather(ops, monkeypatch, name, k):
+    A = MATS[name]
+    n = A.shape[0]
+    X = rng(k + 12).standard_normal((n, k))
+    out = []
+    for layout in ("rows", "columns"):
+        mA = _matrix_with_gather(A, layout, monkeypatch)
+        x = MultiVec(mA, k, F32, data=X)
+        y, y2 = MultiVec(mA, k, F32), MultiVec(mA, k, F32)
+        ops.matdot(mA, x, y)
+        d = ops.matdot_dot(mA, x, y2)
+        np.testing.assert_array_equal(y.get(), y2.get())
+        out.append((y.get(), np.asarray(d)))
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_allclose(out[0][1], out[1][1], rtol=1e-12)
+
+
+def test_spmm_row_gather_column_offsets(ops, monkeypatch):
+    A = MATS["chunglu"]
+    n = A.shape[0]
+    X = rng(13).standard_normal((n, 9))
+    mA = _matrix_with_gather(A, "rows", monkeypatch)
+    x = MultiVec(mA, 9, data=X)
+    y = MultiVec(mA, 12)
+    ops.matdot(mA, x, y, xs=2, ys=5, k=6)
+    Y = y.get()
+    np.testing.assert_array_equal(Y[:, 5:11], O.csr_matmultivec(A, X[:, 2:8]))
+    assert np.all(Y[:, :5] == 0) and np.all(Y[:, 11:] == 0)
+
+
 @pytest.mark.parametrize("a,b", [(0.0, 2.5), (1.0, 0.0), (-3.0, 0.0), (0.7, -1.3), (1.0, 1.0), (0.0, 0.0)])
 def test_axpby_branches_bit_exact(ops, a, b):
     A = MATS["vardiff2d"]
@@ -465,6 +525,32 @@ def test_residual_norms_column_chunks(ops, case):
     np.testing.assert_allclose(rn, np.linalg.norm(R, axis=0), rtol=1e-11)
     np.testing.assert_allclose(xn, np.linalg.norm(X[:, idx], axis=0), rtol=1e-12)
     np.testing.assert_allclose(bn, xn, rtol=0)
+
+
+@pytest.mark.parametrize("name", ["vardiff2d", "chunglu"])
+@pytest.mark.parametrize("prec", [F64, F32])
+@pytest.mark.parametrize("k", [2, 5, 17, 70, 130])
+def test_residual_norms_row_gather(ops, monkeypatch, name, prec, k):
+    # the row-major gather path (B = I): same FP64 row sums as the column
+    # path, so the norms agree with it and with numpy to summation order
+    A = MATS[name]
+    n = A.shape[0]
+    X = rng(k + 50).standard_normal((n, k + 3))
+    idx = rng(k + 51).permutation(k + 3)[:k].astype(np.int32)
+    lam = rng(k + 52).uniform(0.1, 10.0, k)
+    res = []
+    for layout in ("rows", "columns"):
+        mA = _matrix_with_gather(A, layout, monkeypatch)
+        x = MultiVec(mA, k + 3, prec, data=X)
+        res.append(ops.residual_norms(mA, None, x, idx, lam))
+    if prec == F32:
+        X = X.astype(np.float32).astype(np.float64)
+    R = A @ X[:, idx] - X[:, idx] * lam[None, :]
+    np.testing.assert_allclose(res[0][0], np.linalg.norm(R, axis=0), rtol=1e-11)
+    np.testing.assert_allclose(res[0][1], np.linalg.norm(X[:, idx], axis=0), rtol=1e-12)
+    np.testing.assert_allclose(res[0][2], res[0][1], rtol=0)
+    for a, b in zip(res[0], res[1]):
+        np.testing.assert_allclose(a, b, rtol=1e-13)
 
 
 def test_residual_norms_fp32_storage_fp64_math(ops):
