@@ -77,9 +77,15 @@ def kernel_delta(after, before):
 
 
 # profiler class -> the kernel template whose instantiations rocprofv3 lists
+# the SpMM and residual classes time either the column-gather kernel or,
+# for matrices with scattered column indices, the row staging plus the
+# row-gather kernel
+_ROWS = " / k_rows_from_cols + k_spmm_rows<{}, {}, ...>"
 ROCPROF_KERNEL = {
-    "spmm_f64": "k_spmm<double, false, ...>", "spmm_f32": "k_spmm<float, false, ...>",
-    "spmm_dot_f64": "k_spmm<double, true, ...>", "spmm_dot_f32": "k_spmm<float, true, ...>",
+    "spmm_f64": "k_spmm<double, false, ...>" + _ROWS.format("double", "false"),
+    "spmm_f32": "k_spmm<float, false, ...>" + _ROWS.format("float", "false"),
+    "spmm_dot_f64": "k_spmm<double, true, ...>" + _ROWS.format("double", "true"),
+    "spmm_dot_f32": "k_spmm<float, true, ...>" + _ROWS.format("float", "true"),
     "axpby": "k_axpby<...>", "dots": "k_dots<...>",
     "cg_update_f64": "k_cg_resid<double>", "cg_update_f32": "k_cg_resid<float>",
     "cg_search_f64": "k_cg_search<double>", "cg_search_f32": "k_cg_search<float>",
@@ -87,23 +93,24 @@ ROCPROF_KERNEL = {
     "gram_mfma_f32": "k_gram_mfma / k_gram_slab<float, ...>",
     "gram_thin": "k_gram_thin<...>", "lincomb_mfma_f64": "k_lincomb_mfma<double, ...>",
     "lincomb_mfma_f32": "k_lincomb_mfma<float, ...>", "lincomb_thin": "k_lincomb_thin<...>",
-    "residual": "k_residual<...>", "copy": "k_copy<...>", "swap": "k_swap<...>",
+    "residual": "k_residual<...> / k_rows_from_cols + k_residual_rows<...>", "copy": "k_copy<...>", "swap": "k_swap<...>",
 }
 
 
-def load_traffic(run):
+def load_traffic(run, workload):
     """HBM bytes per launch measured with rocprofv3 --pmc for the kernel
-    classes of one run ("mixed" or "fp64") of this workload
-    (profiles/traffic.json)."""
+    classes of one run ("mixed" or "fp64") of the workload they were
+    profiled on (profiles/traffic.json); nothing for another workload."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
-            return json.load(f).get(run, {})
+            d = json.load(f)
+        return d.get(run, {}) if d.get("workload") == workload else {}
     except (OSError, ValueError, AttributeError):
         return {}
 
 
-def roofline(stats, run):
+def roofline(stats, run, workload):
     """Dominant kernel class (largest device time) and its achieved rate:
     algorithmic bytes (or flops, for the MFMA tiles) per launch / average
     launch time, from HIP events on the backend stream."""
@@ -114,7 +121,7 @@ def roofline(stats, run):
     if s["ms"] <= 0:
         return name, None
     cnt = max(1, s["count"])
-    tr = load_traffic(run).get(name)
+    tr = load_traffic(run, workload).get(name)
     common = dict(kernel=ROCPROF_KERNEL.get(name, name), launches=s["count"],
                   avg_ms=round(s["ms"] / cnt, 5), bytes_per_launch=round(s["bytes"] / cnt),
                   traffic=(tr or {}).get("bytes_per_launch"))
@@ -330,7 +337,8 @@ def main():
         except Exception as e:  # the baseline must not hide the GPU result
             log(f"cpu baseline failed: {e}")
 
-    dom, roof = roofline(mixed_stats, "fp64" if args.mode == "fp64" else "mixed")
+    wl = f"{args.workload} {args.size}"
+    dom, roof = roofline(mixed_stats, "fp64" if args.mode == "fp64" else "mixed", wl)
     line = {
         "metric": "Eigenpairs/sec to 1e-8 rel. residual, mixed vs FP64 GCGE, 1 MI355X",
         "value": round(value, 4),
@@ -363,7 +371,7 @@ def main():
         "cpu_baseline": cpu,
     }
     if fp64:
-        d64, roof64 = roofline(fp64["stats"], "fp64")
+        d64, roof64 = roofline(fp64["stats"], "fp64", wl)
         line["fp64"] = dict(eigenpairs_per_s=round(nev / fp64["seconds"], 4), seconds=round(fp64["seconds"], 3),
                             iters=fp64["iters"], converged=fp64["unconverged"] == 0,
                             roofline_kernel=d64, roofline=roof64,

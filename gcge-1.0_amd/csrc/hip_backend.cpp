@@ -457,11 +457,16 @@ extern "C" HIP_MAT *HIP_MatCreateFromCSR(GCGE_INT nrows, GCGE_INT ncols, GCGE_IN
     return A;
 }
 
-// Gather layout of the SpMM.  A wavefront of the column-major kernel reads,
-// per stored entry and column, the cache lines that its 64 rows' column
-// indices fall in; a stencil's rows share a few lines (about 0.03 distinct
-// 32-column segments per entry over a 64-row block), a random graph's do not
-// (0.4 for a Chung-Lu graph).  Above 0.15 the row-major path is used.
+// Gather layout of the SpMM.  The row-major path is used when
+// - the rows average more than 8 stored entries: the column-major kernel
+//   then cannot stage a 256-row workgroup's entries in LDS and each thread
+//   reads its own row's indices and values, 64 cache lines per load (the
+//   27-point mass matrix: 3-7x slower SpMM, 15-20x slower residual check);
+// - or the column indices scatter: a wavefront of the column-major kernel
+//   reads, per stored entry and column, the cache lines its 64 rows' column
+//   indices fall in; a stencil's rows share a few lines (about 0.03 distinct
+//   32-column segments per entry over a 64-row block), a random graph's do
+//   not (0.4 for a Chung-Lu graph); the threshold is 0.15.
 // GCGE_HIP_SPMM_GATHER=rows|columns overrides the choice for matrices
 // created afterwards.
 static int pick_gather_rows(int n, const int *rowptr, const int *col)
@@ -469,6 +474,7 @@ static int pick_gather_rows(int n, const int *rowptr, const int *col)
     const char *e = getenv("GCGE_HIP_SPMM_GATHER");
     if (e && !strcmp(e, "rows")) return 1;
     if (e && !strcmp(e, "columns")) return 0;
+    if ((double)rowptr[n] > 8.0 * n) return 1;
     const int nb = n / 64;
     if (nb == 0) return 0;
     const int step = std::max(1, nb / 2048);

@@ -417,6 +417,66 @@ __device__ __forceinline__ double lane_bcast(double v, int src)
 constexpr int kRowsBatch = 8;       // gathers in flight per lane, short rows
 constexpr int kLongRowsBatch = 32;  // gathers in flight per lane, long rows
 
+// Sum over the CSR row [s, t) of val(e) * Xt(col(e), cc), in CSR order from
+// 0, for a whole wavefront (lane = column): the row's column indices and
+// values are read 64 at a time and broadcast from the lane holding them
+// (readlane), so the gathers have uniform row addresses and kLongRowsBatch
+// of them are in flight.  TA is the accumulation type (T for the SpMM, FP64
+// for the residual check); rfill is a valid row for the idle slots.  Called
+// by all 64 lanes together.
+template <typename TA, typename TV, typename TX>
+__device__ __forceinline__ TA wave_row_sum(const int *__restrict__ col, const TV *__restrict__ val, int s, int t,
+                                           const TX *__restrict__ Xt, int kp, int cc, int rfill, int lane)
+{
+    TA acc = TA(0);
+    for (int e = s; e < t; e += 64) {
+        const int cnt = min(64, t - e);
+        const int cv = lane < cnt ? col[e + lane] : rfill;
+        const TV vv = lane < cnt ? val[e + lane] : TV(0);
+#pragma unroll
+        for (int h = 0; h < 64; h += kLongRowsBatch) {
+            if (h < cnt) {
+                TX xv[kLongRowsBatch];
+#pragma unroll
+                for (int u = 0; u < kLongRowsBatch; u++) {
+                    const int cj = __builtin_amdgcn_readlane(cv, h + u);
+                    xv[u] = Xt[(int64_t)cj * kp + cc];
+                }
+#pragma unroll
+                for (int u = 0; u < kLongRowsBatch; u++)
+                    if (h + u < cnt) acc = acc + lane_bcast(vv, h + u) * (TA)xv[u];
+            }
+        }
+    }
+    return acc;
+}
+
+// The same sum for one lane (row) of a lane group: kRowsBatch entries'
+// gathers in flight; rfill is a valid row for the idle slots.
+template <typename TA, typename TV, typename TX>
+__device__ __forceinline__ TA lane_row_sum(const int *__restrict__ col, const TV *__restrict__ val, int s, int t,
+                                           const TX *__restrict__ Xt, int kp, int c, int rfill)
+{
+    TA acc = TA(0);
+    for (int e = s; e < t; e += kRowsBatch) {
+        const int cnt = t - e;
+        int cj[kRowsBatch];
+        TV v[kRowsBatch];
+        TX xv[kRowsBatch];
+#pragma unroll
+        for (int u = 0; u < kRowsBatch; u++) {
+            cj[u] = u < cnt ? col[e + u] : rfill;
+            v[u] = u < cnt ? val[e + u] : TV(0);
+        }
+#pragma unroll
+        for (int u = 0; u < kRowsBatch; u++) xv[u] = Xt[(int64_t)cj[u] * kp + c];
+#pragma unroll
+        for (int u = 0; u < kRowsBatch; u++)
+            if (u < cnt) acc = acc + v[u] * (TA)xv[u];
+    }
+    return acc;
+}
+
 // Rows handled by one workgroup of k_spmm_rows<G>: four wavefronts of
 // max(16, 64 / G) rows each.
 __host__ __device__ constexpr int rows_per_wg(int G) { return 4 * (64 / G > 16 ? 64 / G : 16); }
@@ -457,26 +517,7 @@ __global__ __launch_bounds__(256) void k_spmm_rows(int n, const int *__restrict_
         const int c = c0 + lane;
         const bool live = c < k;
         const int cc = min(c, kp - 1);
-        T acc = T(0);
-        for (int e = s; e < t; e += 64) {
-            const int cnt = min(64, t - e);
-            const int cv = lane < cnt ? col[e + lane] : r;
-            const T vv = lane < cnt ? val[e + lane] : T(0);
-#pragma unroll
-            for (int h = 0; h < 64; h += kLongRowsBatch) {
-                if (h < cnt) {
-                    T xv[kLongRowsBatch];
-#pragma unroll
-                    for (int u = 0; u < kLongRowsBatch; u++) {
-                        const int cj = __builtin_amdgcn_readlane(cv, h + u);
-                        xv[u] = Xt[(int64_t)cj * kp + cc];
-                    }
-#pragma unroll
-                    for (int u = 0; u < kLongRowsBatch; u++)
-                        if (h + u < cnt) acc = acc + lane_bcast(vv, h + u) * xv[u];
-                }
-            }
-        }
+        const T acc = wave_row_sum<T, T, T>(col, val, s, t, Xt, kp, cc, r, lane);
         if (live) {
             Y[(int64_t)c * ldy + r] = acc;
             if (DOT) partial[(int64_t)c * pstride + nblk + li] = (double)Xt[(int64_t)r * kp + cc] * (double)acc;
@@ -499,22 +540,7 @@ __global__ __launch_bounds__(256) void k_spmm_rows(int n, const int *__restrict_
         const bool act = r < n && t - s <= long_thr;
         if (!act) t = s;
         const int rs = r < n ? r : 0;
-        T acc = T(0);
-        for (int e = s; e < t; e += kRowsBatch) {
-            const int cnt = t - e;
-            int cj[kRowsBatch];
-            T v[kRowsBatch], xv[kRowsBatch];
-#pragma unroll
-            for (int u = 0; u < kRowsBatch; u++) {
-                cj[u] = u < cnt ? col[e + u] : rs;
-                v[u] = u < cnt ? val[e + u] : T(0);
-            }
-#pragma unroll
-            for (int u = 0; u < kRowsBatch; u++) xv[u] = Xt[(int64_t)cj[u] * kp + c];
-#pragma unroll
-            for (int u = 0; u < kRowsBatch; u++)
-                if (u < cnt) acc = acc + v[u] * xv[u];
-        }
+        const T acc = lane_row_sum<T, T, T>(col, val, s, t, Xt, kp, c, rs);
         if (cl == 0) skip[rl] = act ? 0 : 1;
         sY[cl][rl] = acc;
         if (DOT && act && live) dsum += (double)Xt[(int64_t)r * kp + c] * (double)acc;
@@ -638,16 +664,19 @@ __global__ __launch_bounds__(kSpmmRows) void k_residual(int n, const int *__rest
         partial[(int64_t)q * nb + blk] = (sred[q][0] + sred[q][1]) + (sred[q][2] + sred[q][3]);
 }
 
-// r = A x - lambda x (B = I) on the row-major gather path: the FP64 row sums
-// of k_residual (CSR order from 0, x converted to FP64 per entry) gathered
-// from the staged rows Xt, in the workgroup layout of k_spmm_rows (long rows
-// one wavefront each).  The sums r.r, x.x and Bx.Bx (= x.x) of column c go
-// to partial[(3c + q) * pstride + blk] per row workgroup and to
+// r = A x - lambda B x (Bx = x when B is absent) on the row-major gather
+// path: the FP64 row sums of k_residual (CSR order from 0, x converted to
+// FP64 per entry) gathered from the staged rows Xt, in the workgroup layout
+// of k_spmm_rows (A's long rows one wavefront each; B has no row longer than
+// long_thr).  The sums r.r, x.x and Bx.Bx of column c go to
+// partial[(3c + q) * pstride + blk] per row workgroup and to
 // partial[(3c + q) * pstride + nblk + li] per long row.
-template <typename TX, int G>
+template <typename TX, int G, bool HASB>
 __global__ __launch_bounds__(256) void k_residual_rows(int n, const int *__restrict__ rowptr,
                                                        const int *__restrict__ col, const double *__restrict__ val,
-                                                       const TX *__restrict__ Xt, int kp, int k, ColScalars g,
+                                                       const int *__restrict__ brp, const int *__restrict__ bcol,
+                                                       const double *__restrict__ bval, const TX *__restrict__ Xt,
+                                                       int kp, int k, ColScalars g,
                                                        const int *__restrict__ long_rows, int nlong, int nlb,
                                                        int long_thr, double *__restrict__ partial, int pstride,
                                                        int nblk)
@@ -655,42 +684,24 @@ __global__ __launch_bounds__(256) void k_residual_rows(int n, const int *__restr
     constexpr int SG = 64 / G;
     constexpr int RWG = rows_per_wg(G);
     constexpr int RPW = RWG / 4;
-    __shared__ double sred[4][G][2];
+    __shared__ double sred[4][G][3];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int c0 = blockIdx.y * 64;
     if ((int)blockIdx.x < nlb) {
         const int li = blockIdx.x * 4 + wid;
         if (li >= nlong) return;  // uniform per wavefront; this branch has no barrier
         const int r = long_rows[li];
-        const int s = rowptr[r], t = rowptr[r + 1];
         const int c = c0 + lane;
         const int cc = min(c, kp - 1);
-        double ax = 0.0;
-        for (int e = s; e < t; e += 64) {
-            const int cnt = min(64, t - e);
-            const int cv = lane < cnt ? col[e + lane] : r;
-            const double vv = lane < cnt ? val[e + lane] : 0.0;
-#pragma unroll
-            for (int h = 0; h < 64; h += kLongRowsBatch) {
-                if (h < cnt) {
-                    TX xv[kLongRowsBatch];
-#pragma unroll
-                    for (int u = 0; u < kLongRowsBatch; u++) {
-                        const int cj = __builtin_amdgcn_readlane(cv, h + u);
-                        xv[u] = Xt[(int64_t)cj * kp + cc];
-                    }
-#pragma unroll
-                    for (int u = 0; u < kLongRowsBatch; u++)
-                        if (h + u < cnt) ax = ax + lane_bcast(vv, h + u) * (double)xv[u];
-                }
-            }
-        }
+        const double ax = wave_row_sum<double, double, TX>(col, val, rowptr[r], rowptr[r + 1], Xt, kp, cc, r, lane);
+        const double xr = (double)Xt[(int64_t)r * kp + cc];
+        const double bx =
+            HASB ? wave_row_sum<double, double, TX>(bcol, bval, brp[r], brp[r + 1], Xt, kp, cc, r, lane) : xr;
         if (c < k) {
-            const double xr = (double)Xt[(int64_t)r * kp + cc];
-            const double res = axpby_el<double>(-g.a[c], xr, 1.0, ax);
+            const double res = axpby_el<double>(-g.a[c], bx, 1.0, ax);
             partial[(int64_t)(3 * c + 0) * pstride + nblk + li] = res * res;
             partial[(int64_t)(3 * c + 1) * pstride + nblk + li] = xr * xr;
-            partial[(int64_t)(3 * c + 2) * pstride + nblk + li] = xr * xr;
+            partial[(int64_t)(3 * c + 2) * pstride + nblk + li] = bx * bx;
         }
         return;
     }
@@ -699,59 +710,51 @@ __global__ __launch_bounds__(256) void k_residual_rows(int n, const int *__restr
     const int cl = lane % G, sg = lane / G;
     const int c = c0 + cl;  // < kp
     const bool live = c < k;
-    double srr = 0.0, sxx = 0.0;
+    double srr = 0.0, sxx = 0.0, sbb = 0.0;
     for (int i = sg; i < RPW; i += SG) {
         const int r = r0 + wid * RPW + i;
-        int s = 0, t = 0;
+        int s = 0, t = 0, bs = 0, bt = 0;
         if (r < n) {
             s = rowptr[r];
             t = rowptr[r + 1];
+            if (HASB) {
+                bs = brp[r];
+                bt = brp[r + 1];
+            }
         }
         const bool act = r < n && t - s <= long_thr;
-        if (!act) t = s;
+        if (!act) t = s, bt = bs;
         const int rs = r < n ? r : 0;
-        double ax = 0.0;
-        for (int e = s; e < t; e += kRowsBatch) {
-            const int cnt = t - e;
-            int cj[kRowsBatch];
-            double v[kRowsBatch];
-            TX xv[kRowsBatch];
-#pragma unroll
-            for (int u = 0; u < kRowsBatch; u++) {
-                cj[u] = u < cnt ? col[e + u] : rs;
-                v[u] = u < cnt ? val[e + u] : 0.0;
-            }
-#pragma unroll
-            for (int u = 0; u < kRowsBatch; u++) xv[u] = Xt[(int64_t)cj[u] * kp + c];
-#pragma unroll
-            for (int u = 0; u < kRowsBatch; u++)
-                if (u < cnt) ax = ax + v[u] * (double)xv[u];
-        }
+        const double ax = lane_row_sum<double, double, TX>(col, val, s, t, Xt, kp, c, rs);
+        const double bx0 = HASB ? lane_row_sum<double, double, TX>(bcol, bval, bs, bt, Xt, kp, c, rs) : 0.0;
         if (act && live) {
             const double xr = (double)Xt[(int64_t)r * kp + c];
-            const double res = axpby_el<double>(-g.a[c], xr, 1.0, ax);
+            const double bx = HASB ? bx0 : xr;
+            const double res = axpby_el<double>(-g.a[c], bx, 1.0, ax);
             srr += res * res;
             sxx += xr * xr;
+            sbb += bx * bx;
         }
     }
 #pragma unroll
     for (int m = G; m < 64; m <<= 1) {
         srr += __shfl_xor(srr, m, 64);
         sxx += __shfl_xor(sxx, m, 64);
+        sbb += __shfl_xor(sbb, m, 64);
     }
     if (sg == 0) {
         sred[wid][cl][0] = srr;
         sred[wid][cl][1] = sxx;
+        sred[wid][cl][2] = sbb;
     }
     __syncthreads();
     const int q = threadIdx.x;
     if (q < min(G, k - c0)) {
         const int cq = c0 + q;
-        const double rr = (sred[0][q][0] + sred[1][q][0]) + (sred[2][q][0] + sred[3][q][0]);
-        const double xx = (sred[0][q][1] + sred[1][q][1]) + (sred[2][q][1] + sred[3][q][1]);
-        partial[(int64_t)(3 * cq + 0) * pstride + blk] = rr;
-        partial[(int64_t)(3 * cq + 1) * pstride + blk] = xx;
-        partial[(int64_t)(3 * cq + 2) * pstride + blk] = xx;
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            partial[(int64_t)(3 * cq + j) * pstride + blk] =
+                (sred[0][q][j] + sred[1][q][j]) + (sred[2][q][j] + sred[3][q][j]);
     }
 }
 
@@ -1158,10 +1161,11 @@ bool spmm_dot(int prec, HIP_MAT *A, const void *X, int64_t ldx, void *Y, int64_t
     return spmm_t<float, true>(A, X, ldx, Y, ldy, k, nullptr, out);
 }
 
-// Residual sums of kc <= kMaxCols columns on the row-major gather path (B = I);
-// out[3j + q]: r.r, x.x, x.x of column j.
+// Residual sums of kc <= kMaxCols columns on the row-major gather path;
+// out[3j + q]: r.r, x.x, Bx.Bx of column j.  B, when given, has no row longer
+// than kLongRow.
 template <typename T, int G>
-static bool residual_rows_g(HIP_MAT *A, const T *X, int64_t ldx, ColScalars g, int kc, double *out)
+static bool residual_rows_g(HIP_MAT *A, HIP_MAT *B, const T *X, int64_t ldx, ColScalars g, int kc, double *out)
 {
     const int n = A->n, kp = G < 64 ? G : ((kc + 63) / 64) * 64;
     T *Xt = (T *)row_stage(sizeof(T) * (size_t)n * kp);
@@ -1172,27 +1176,36 @@ static bool residual_rows_g(HIP_MAT *A, const T *X, int64_t ldx, ColScalars g, i
     double *partial = (double *)scratch(sizeof(double) * ((size_t)3 * kc * pstride + 3 * kc));
     if (!partial) return false;
     {
-        ProfScope ps(K_RESIDUAL, 12.0 * A->nnz + 4.0 * (n + 1) + (double)kc * n * sizeof(T),
-                     2.0 * kc * (double)A->nnz);
+        ProfScope ps(K_RESIDUAL,
+                     12.0 * A->nnz + 4.0 * (n + 1) + (B ? 12.0 * B->nnz + 4.0 * (n + 1) : 0.0) +
+                         (double)kc * n * sizeof(T),
+                     2.0 * kc * (double)(A->nnz + (B ? B->nnz : 0)));
         k_rows_from_cols<T><<<dim3((n + 63) / 64, (kp + 63) / 64), 256, 0, ctx().stream>>>(n, X, ldx, kc, g, kp,
                                                                                            Xt);
-        k_residual_rows<T, G><<<dim3(nlb + nblk, G < 64 ? 1 : kp / 64), 256, 0, ctx().stream>>>(
-            n, A->rowptr, A->col, A->val64, Xt, kp, kc, g, A->long_rows, nlong, nlb,
-            nlong > 0 ? kLongRow : INT_MAX, partial, pstride, nblk);
+        const dim3 grid(nlb + nblk, G < 64 ? 1 : kp / 64);
+        const int lthr = nlong > 0 ? kLongRow : INT_MAX;
+        if (B)
+            k_residual_rows<T, G, true><<<grid, 256, 0, ctx().stream>>>(
+                n, A->rowptr, A->col, A->val64, B->rowptr, B->col, B->val64, Xt, kp, kc, g, A->long_rows, nlong,
+                nlb, lthr, partial, pstride, nblk);
+        else
+            k_residual_rows<T, G, false><<<grid, 256, 0, ctx().stream>>>(
+                n, A->rowptr, A->col, A->val64, nullptr, nullptr, nullptr, Xt, kp, kc, g, A->long_rows, nlong, nlb,
+                lthr, partial, pstride, nblk);
     }
     if (!check(hipGetLastError(), "k_residual_rows")) return false;
     return reduce_to_host(partial, 3 * kc, pstride, out);
 }
 
 template <typename T>
-static bool residual_rows(HIP_MAT *A, const T *X, int64_t ldx, ColScalars g, int kc, double *out)
+static bool residual_rows(HIP_MAT *A, HIP_MAT *B, const T *X, int64_t ldx, ColScalars g, int kc, double *out)
 {
-    if (kc > 32) return residual_rows_g<T, 64>(A, X, ldx, g, kc, out);
-    if (kc > 16) return residual_rows_g<T, 32>(A, X, ldx, g, kc, out);
-    if (kc > 8) return residual_rows_g<T, 16>(A, X, ldx, g, kc, out);
-    if (kc > 4) return residual_rows_g<T, 8>(A, X, ldx, g, kc, out);
-    if (kc > 2) return residual_rows_g<T, 4>(A, X, ldx, g, kc, out);
-    return residual_rows_g<T, 2>(A, X, ldx, g, kc, out);
+    if (kc > 32) return residual_rows_g<T, 64>(A, B, X, ldx, g, kc, out);
+    if (kc > 16) return residual_rows_g<T, 32>(A, B, X, ldx, g, kc, out);
+    if (kc > 8) return residual_rows_g<T, 16>(A, B, X, ldx, g, kc, out);
+    if (kc > 4) return residual_rows_g<T, 8>(A, B, X, ldx, g, kc, out);
+    if (kc > 2) return residual_rows_g<T, 4>(A, B, X, ldx, g, kc, out);
+    return residual_rows_g<T, 2>(A, B, X, ldx, g, kc, out);
 }
 
 template <typename T>
@@ -1204,8 +1217,9 @@ static bool residual_t(HIP_MAT *A, HIP_MAT *B, const void *X, int64_t ldx, const
     for (int c0 = 0; c0 < k; c0 += kMaxCols) {
         const int kc = std::min(kMaxCols, k - c0);
         ColScalars g = make_scalars(kc, lambda + c0, nullptr, idx + c0);
-        if (!B && A->gather_rows && kc >= 2) {
-            if (!residual_rows<T>(A, (const T *)X, ldx, g, kc, tmp.data())) return false;
+        // row gather when either matrix prefers it (B's rows must all be short)
+        if ((A->gather_rows || (B && B->gather_rows)) && (!B || B->nlong == 0) && kc >= 2) {
+            if (!residual_rows<T>(A, B, (const T *)X, ldx, g, kc, tmp.data())) return false;
             for (int j = 0; j < kc; j++) {
                 rnorm[c0 + j] = std::sqrt(tmp[3 * j + 0]);
                 xnorm[c0 + j] = std::sqrt(tmp[3 * j + 1]);

@@ -44,13 +44,22 @@ def mats():
 MATS = mats()
 
 
+def _matrix_with_gather(A, layout, monkeypatch):
+    # the SpMM gather layout is chosen when the matrix is created
+    monkeypatch.setenv("GCGE_HIP_SPMM_GATHER", layout)
+    m = Matrix(A)
+    monkeypatch.delenv("GCGE_HIP_SPMM_GATHER")
+    return m
+
+
 @pytest.mark.parametrize("name", list(MATS))
 @pytest.mark.parametrize("k", [1, 3, 4, 70])
-def test_spmm_fp64_bit_exact(ops, name, k):
+def test_spmm_fp64_bit_exact(ops, monkeypatch, name, k):
+    # column-gather path (the row-gather path has its own tests below)
     A = MATS[name]
     n = A.shape[0]
     X = rng(k).standard_normal((n, k))
-    mA = Matrix(A)
+    mA = _matrix_with_gather(A, "columns", monkeypatch)
     x = MultiVec(mA, k, data=X)
     y = MultiVec(mA, k)
     ops.matdot(mA, x, y)
@@ -58,11 +67,12 @@ def test_spmm_fp64_bit_exact(ops, name, k):
 
 
 @pytest.mark.parametrize("name", ["vardiff2d", "mass27", "chunglu"])
-def test_spmm_dot_fused(ops, name):
+@pytest.mark.parametrize("layout", ["columns", "rows"])
+def test_spmm_dot_fused(ops, monkeypatch, name, layout):
     A = MATS[name]
     n, k = A.shape[0], 6
     X = rng(3).standard_normal((n, k))
-    mA = Matrix(A)
+    mA = _matrix_with_gather(A, layout, monkeypatch)
     x = MultiVec(mA, k, data=X)
     y = MultiVec(mA, k)
     d = ops.matdot_dot(mA, x, y)
@@ -94,14 +104,6 @@ def test_spmm_column_offsets(ops):
     Y = y.get()
     np.testing.assert_array_equal(Y[:, 5:11], O.csr_matmultivec(A, X[:, 2:8]))
     assert np.all(Y[:, :5] == 0) and np.all(Y[:, 11:] == 0)
-
-
-def _matrix_with_gather(A, layout, monkeypatch):
-    # the SpMM gather layout is chosen when the matrix is created
-    monkeypatch.setenv("GCGE_HIP_SPMM_GATHER", layout)
-    m = Matrix(A)
-    monkeypatch.delenv("GCGE_HIP_SPMM_GATHER")
-    return m
 
 
 @pytest.mark.parametrize("name", list(MATS))
@@ -550,6 +552,37 @@ def test_residual_norms_row_gather(ops, monkeypatch, name, prec, k):
     np.testing.assert_allclose(res[0][1], np.linalg.norm(X[:, idx], axis=0), rtol=1e-12)
     np.testing.assert_allclose(res[0][2], res[0][1], rtol=0)
     for a, b in zip(res[0], res[1]):
+        np.testing.assert_allclose(a, b, rtol=1e-13)
+
+
+@pytest.mark.parametrize("prec", [F64, F32])
+@pytest.mark.parametrize("k", [3, 40])
+@pytest.mark.parametrize("layout", ["rows", "columns"])
+def test_residual_norms_row_gather_with_b(ops, monkeypatch, prec, k, layout):
+    # generalized problem: B (27-point mass) on the row-gather path makes the
+    # residual check gather both A's and B's rows from the staged X rows
+    A, B = P.lap3d_mass(9)
+    n = A.shape[0]
+    X = rng(k + 60).standard_normal((n, k + 2))
+    idx = rng(k + 61).permutation(k + 2)[:k].astype(np.int32)
+    lam = rng(k + 62).uniform(1.0, 100.0, k)
+    mA = _matrix_with_gather(A, layout, monkeypatch)
+    mB = _matrix_with_gather(B, "rows", monkeypatch)
+    x = MultiVec(mA, k + 2, prec, data=X)
+    rn, xn, bn = ops.residual_norms(mA, mB, x, idx, lam)
+    if prec == F32:
+        X = X.astype(np.float32).astype(np.float64)
+    BX = B @ X[:, idx]
+    R = A @ X[:, idx] - BX * lam[None, :]
+    np.testing.assert_allclose(rn, np.linalg.norm(R, axis=0), rtol=1e-11)
+    np.testing.assert_allclose(xn, np.linalg.norm(X[:, idx], axis=0), rtol=1e-12)
+    np.testing.assert_allclose(bn, np.linalg.norm(BX, axis=0), rtol=1e-12)
+    # and the column path of the same pair agrees to summation order
+    mB2 = _matrix_with_gather(B, "columns", monkeypatch)
+    mA2 = _matrix_with_gather(A, "columns", monkeypatch)
+    x2 = MultiVec(mA2, k + 2, prec, data=X)
+    ref = ops.residual_norms(mA2, mB2, x2, idx, lam)
+    for a, b in zip((rn, xn, bn), ref):
         np.testing.assert_allclose(a, b, rtol=1e-13)
 
 
