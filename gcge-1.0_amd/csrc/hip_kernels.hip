@@ -371,8 +371,10 @@ __global__ __launch_bounds__(kSpmmRows) void k_spmm_long(const int *__restrict__
 
 // ------------------------------------------------- SpMM, row-major gather
 // Matrices whose column indices scatter (random graphs) make the column-major
-// gather of k_spmm touch one cache line per stored entry and column.  For
-// them (HIP_MAT::gather_rows) the k columns are first copied into a
+// gather of k_spmm touch one cache line per stored entry and column, and
+// matrices whose rows average more than 8 entries overflow its LDS stage, so
+// that each thread reads its own row's indices and values.  For both
+// (HIP_MAT::gather_rows, chosen at creation) the k columns are first copied into a
 // row-major staging block Xt, row r holding X(r, 0..kp) (kp: k rounded up to
 // the lane group G), and k_spmm_rows gathers whole row segments: G lanes of a
 // wavefront take G consecutive columns of one row, so a stored entry costs
