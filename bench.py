@@ -320,13 +320,13 @@ def main():
         L.GCGE_HIP_Synchronize()
         t64s = time.perf_counter()
         r64s = solve(mA, mB, nev, X0=X0, precision="fp64", keep_evec=False, w_orth_type="cholqr",
-                     cg_max_it=args.cg_max_it, bcg_precond=args.bcg_precond, **common)
+                     cg_max_it=args.cg_max_it, bcg_precond=args.bcg_precond, rr_eig_full=1, **common)
         L.GCGE_HIP_Synchronize()
         t64s = time.perf_counter() - t64s
         fp64s = dict(seconds=t64s, iters=r64s["num_iter"], unconverged=r64s["num_unlock"],
                      eval=r64s["eval"])
         log(f"fp64 with the mixed path's settings (CholQR, cg_max_it {args.cg_max_it}, bcg_precond "
-            f"{args.bcg_precond}): {t64s:.2f}s, "
+            f"{args.bcg_precond}, dsyevd Rayleigh-Ritz): {t64s:.2f}s, "
             f"{r64s['num_iter']} it, unconverged {r64s['num_unlock']}")
 
     cpu = None
@@ -358,6 +358,7 @@ def main():
                    "criterion": "||Ax-lambda Bx||/(|lambda| ||Bx||)", "x0": "U[-1,1] seeded",
                    "mode": args.mode, "fp32_tol": args.fp32_tol, "cg_max_it": args.cg_max_it,
                    "bcg_precond": "jacobi" if args.bcg_precond else "none",
+                   "rr_eig": "dsyevd" if args.mode == "mixed" else "dsyevx",
                    "stagnation": [args.stagnation_window, 0.7, args.stagnation_start]},
         "converged": bool(converged),
         "max_ns_residual": float(np.max(ns)),
@@ -384,11 +385,13 @@ def main():
                                                            np.abs(fp64["eval"])))
     if fp64s:
         # the precision-only comparison: FP64 with every algorithm setting of
-        # the mixed path (CholQR, inner CG step cap)
+        # the mixed path (CholQR, inner CG step cap and preconditioner, dense
+        # Rayleigh-Ritz solver)
         line["fp64_same_settings"] = dict(eigenpairs_per_s=round(nev / fp64s["seconds"], 4),
                                           seconds=round(fp64s["seconds"], 3), iters=fp64s["iters"],
                                           converged=fp64s["unconverged"] == 0, w_orth_type="cholqr",
-                                          cg_max_it=args.cg_max_it, bcg_precond=args.bcg_precond)
+                                          cg_max_it=args.cg_max_it, bcg_precond=args.bcg_precond,
+                                          rr_eig="dsyevd")
         line["mixed_vs_fp64_same_settings"] = round(fp64s["seconds"] / (ms_per_step * 1e-3), 3)
     print(json.dumps(line), flush=True)
 

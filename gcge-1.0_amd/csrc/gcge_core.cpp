@@ -284,6 +284,7 @@ void stable_multi_orth(void **V, int start, int *end, void *B, GCGE_OPS *ops, GC
 
 int gcge_chol_inverse(int n, double *S, int lds);            // gcge_dense.cpp
 int gcge_sym_eig(int n, double *S, int lds, double *w, double *Z);
+int gcge_sym_eig_range(int n, const double *S, int lds, int il, int iu, double *w, double *Z, int ldz);
 
 namespace {
 
@@ -862,11 +863,16 @@ void rr_eigenpairs(double *subspace_matrix, double *eval, double *subspace_evec,
     int nrows = ldm - rr_eigen_start, ncols = ldm;
     il -= rr_eigen_start;
     iu -= rr_eigen_start;
-    ops->DenseMatEigenSolver((char *)"V", (char *)"I", (char *)"U", &nrows,
-                             temp + (size_t)rr_eigen_start * ncols + rr_eigen_start, &ncols, &vl, &vu, &il, &iu,
-                             &abstol, &m, eval + rr_eigen_start,
-                             subspace_evec + (size_t)rr_eigen_start * ldm + rr_eigen_start, &ldm, isuppz, dwork,
-                             &lwork, ws->subspace_itmp, &liwork, ifail, &info);
+    if (para->rr_eig_full)
+        info = gcge_sym_eig_range(nrows, temp + (size_t)rr_eigen_start * ncols + rr_eigen_start, ncols, il, iu,
+                                  eval + rr_eigen_start, subspace_evec + (size_t)rr_eigen_start * ldm + rr_eigen_start,
+                                  ldm);
+    else
+        ops->DenseMatEigenSolver((char *)"V", (char *)"I", (char *)"U", &nrows,
+                                 temp + (size_t)rr_eigen_start * ncols + rr_eigen_start, &ncols, &vl, &vu, &il, &iu,
+                                 &abstol, &m, eval + rr_eigen_start,
+                                 subspace_evec + (size_t)rr_eigen_start * ldm + rr_eigen_start, &ldm, isuppz, dwork,
+                                 &lwork, ws->subspace_itmp, &liwork, ifail, &info);
     if (info != 0) fprintf(stderr, "GCGE: dense eigensolver returned info = %d\n", info);
 }
 

@@ -33,6 +33,8 @@ typedef void (*copy_t)(int *, double *, int *, double *, int *);
 typedef void (*potrf_t)(const char *, int *, double *, int *, int *);
 typedef void (*trtri_t)(const char *, const char *, int *, double *, int *, int *);
 typedef void (*syev_t)(const char *, const char *, int *, double *, int *, double *, double *, int *, int *);
+typedef void (*syevd_t)(const char *, const char *, int *, double *, int *, double *, double *, int *, int *, int *,
+                        int *);
 
 struct Lapack {
     bool ok = false;
@@ -47,6 +49,7 @@ struct Lapack {
     potrf_t dpotrf_ = nullptr;  // used by the CholQR orthonormalization only
     trtri_t dtrtri_ = nullptr;
     syev_t dsyev_ = nullptr;    // fallback of the subspace eigensolver (see d_eig)
+    syevd_t dsyevd_ = nullptr;  // full decomposition for para->rr_eig_full
 };
 
 Lapack &lapack()
@@ -73,6 +76,7 @@ Lapack &lapack()
     L.dpotrf_ = (potrf_t)dlsym(h, "dpotrf_");
     L.dtrtri_ = (trtri_t)dlsym(h, "dtrtri_");
     L.dsyev_ = (syev_t)dlsym(h, "dsyev_");
+    L.dsyevd_ = (syevd_t)dlsym(h, "dsyevd_");
     L.ok = L.dsyevx_ && L.dgemm_ && L.dsymm_ && L.ddot_ && L.dnrm2_ && L.dscal_ && L.daxpy_ && L.dcopy_;
     if (!L.ok) fprintf(stderr, "GCGE: LAPACK library %s lacks a required routine\n", name);
     return L;
@@ -238,4 +242,44 @@ int gcge_sym_eig(int n, double *S, int lds, double *w, double *Z)
     free(iwork);
     free(ifail);
     return info;
+}
+
+// Eigenpairs il..iu (1-based, ascending) of the n x n symmetric matrix S
+// (column-major, leading dimension lds, upper triangle read; S is left
+// unchanged) from its full decomposition by dsyevd (divide and conquer):
+// eigenvalues to w[0 .. iu-il], eigenvectors to the columns of Z (leading
+// dimension ldz).  Falls back to dsyevx with range "I" when the library has
+// no dsyevd.  Returns LAPACK's info (0 on success), -1 without LAPACK.
+int gcge_sym_eig_range(int n, const double *S, int lds, int il, int iu, double *w, double *Z, int ldz)
+{
+    Lapack &L = lapack();
+    if (!L.ok || n <= 0 || il < 1 || iu > n || il > iu) return -1;
+    thread_local std::vector<double> a, wa, work;
+    thread_local std::vector<int> iwork;
+    a.resize((size_t)n * n);
+    for (int j = 0; j < n; j++) std::memcpy(a.data() + (size_t)j * n, S + (size_t)j * lds, sizeof(double) * n);
+    int nn = n, lda = n, info = 0;
+    const int m = iu - il + 1;
+    if (!L.dsyevd_) {
+        double vl = 0.0, vu = 0.0, abstol = 0.0;
+        int mm = 0, lw = 8 * n, ldzz = ldz, ilo = il, ihi = iu;
+        work.resize(lw);
+        iwork.resize(6 * (size_t)n);
+        wa.resize(n);
+        L.dsyevx_("V", "I", "U", &nn, a.data(), &lda, &vl, &vu, &ilo, &ihi, &abstol, &mm, wa.data(), Z, &ldzz,
+                  work.data(), &lw, iwork.data(), iwork.data() + 5 * (size_t)n, &info);
+        std::memcpy(w, wa.data(), sizeof(double) * m);
+        return info;
+    }
+    int lwork = 1 + 6 * n + 2 * n * n, liwork = 3 + 5 * n;
+    work.resize(lwork);
+    iwork.resize(liwork);
+    wa.resize(n);
+    L.dsyevd_("V", "U", &nn, a.data(), &lda, wa.data(), work.data(), &lwork, iwork.data(), &liwork, &info);
+    if (info != 0) return info;
+    for (int j = 0; j < m; j++) {
+        w[j] = wa[il - 1 + j];
+        std::memcpy(Z + (size_t)j * ldz, a.data() + (size_t)(il - 1 + j) * n, sizeof(double) * n);
+    }
+    return 0;
 }

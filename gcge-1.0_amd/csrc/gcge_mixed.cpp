@@ -45,7 +45,8 @@ extern "C" GCGE_INT GCGE_HIP_SOLVER_SolveMixed(GCGE_SOLVER *solver, const GCGE_H
     o.fp64_max_it = (opts && opts->fp64_max_it > 0) ? opts->fp64_max_it : P->ev_max_it;
     // without opts: the configuration of the mixed path (FP32 phase with CholQR,
     // stopped at its stagnation; refinement with CholQR and FP32 inner solves;
-    // Jacobi-preconditioned inner solves of up to 30 CG steps in both phases)
+    // Jacobi-preconditioned inner solves of up to 30 CG steps in both phases;
+    // Rayleigh-Ritz pairs from the full dsyevd decomposition)
     o.stagnation_window = opts ? opts->stagnation_window : 10;
     o.stagnation_ratio = (opts && opts->stagnation_ratio > 0) ? opts->stagnation_ratio : 0.7;
     o.fp32_w_orth = opts ? opts->fp32_w_orth : 1;
@@ -54,6 +55,7 @@ extern "C" GCGE_INT GCGE_HIP_SOLVER_SolveMixed(GCGE_SOLVER *solver, const GCGE_H
     o.stagnation_start = opts ? opts->stagnation_start : 1e-3;
     o.cg_max_it = opts ? (opts->cg_max_it > 0 ? opts->cg_max_it : P->cg_max_it) : 30;
     o.bcg_precond = opts ? (opts->bcg_precond > 0 ? opts->bcg_precond : P->bcg_precond) : 1;
+    o.rr_eig_full = opts ? (opts->rr_eig_full > 0 ? opts->rr_eig_full : P->rr_eig_full) : 1;
     GCGE_HIP_MIXED_INFO inf;
     std::memset(&inf, 0, sizeof(inf));
 
@@ -86,6 +88,7 @@ extern "C" GCGE_INT GCGE_HIP_SOLVER_SolveMixed(GCGE_SOLVER *solver, const GCGE_H
         s32->para->basis_eps = FLT_EPSILON;
         s32->para->cg_max_it = o.cg_max_it;
         s32->para->bcg_precond = o.bcg_precond;
+        s32->para->rr_eig_full = o.rr_eig_full;
         if (o.fp32_w_orth) s32->para->w_orth_type = (char *)"cholqr";
     }
     s32->eval = (double *)calloc(nev, sizeof(double));
@@ -116,13 +119,14 @@ extern "C" GCGE_INT GCGE_HIP_SOLVER_SolveMixed(GCGE_SOLVER *solver, const GCGE_H
     GCGE_SOLVER_Free_All(&s32);
     const double t1 = now_s();
     const int given = P->given_init_evec, maxit = P->ev_max_it, bcgp = P->bcg_precision, cgit = P->cg_max_it,
-              pcnd = P->bcg_precond;
+              pcnd = P->bcg_precond, rrf = P->rr_eig_full;
     char *worth = P->w_orth_type;
     P->given_init_evec = 1;
     P->ev_max_it = o.fp64_max_it;
     P->bcg_precision = o.refine_bcg_precision;
     P->cg_max_it = o.cg_max_it;
     P->bcg_precond = o.bcg_precond;
+    P->rr_eig_full = o.rr_eig_full;
     if (o.refine_w_orth) P->w_orth_type = (char *)"cholqr";
     P->num_iter = 0;
     P->num_unlock = nev;
@@ -143,6 +147,7 @@ extern "C" GCGE_INT GCGE_HIP_SOLVER_SolveMixed(GCGE_SOLVER *solver, const GCGE_H
     P->bcg_precision = bcgp;
     P->cg_max_it = cgit;
     P->bcg_precond = pcnd;
+    P->rr_eig_full = rrf;
     P->w_orth_type = worth;
     inf.fp64_iters = P->num_iter;
     inf.fp64_seconds = now_s() - t1;

@@ -89,6 +89,7 @@ extern "C" void GCGE_PARA_Create(GCGE_PARA **para)
     p->lock_converged = 1;
     p->basis_eps = DBL_EPSILON;
     p->bcg_precond = 0;
+    p->rr_eig_full = 0;
     *para = p;
 }
 
@@ -149,6 +150,7 @@ extern "C" GCGE_INT GCGE_PARA_SetFromCommandLine(GCGE_PARA *p, GCGE_INT argc, ch
         else if (is("-gcge_bcg_precision")) p->bcg_precision = atoi(v);
         else if (is("-gcge_lock_converged")) p->lock_converged = atoi(v);
         else if (is("-gcge_bcg_precond")) p->bcg_precond = atoi(v);
+        else if (is("-gcge_rr_eig_full")) p->rr_eig_full = atoi(v);
         else if (std::strcmp(a, "-gcge_help") == 0) {
             usage = 1;
             break;
@@ -164,7 +166,8 @@ extern "C" GCGE_INT GCGE_PARA_SetFromCommandLine(GCGE_PARA *p, GCGE_INT argc, ch
                "       [-gcge_orth_zero_tol d] [-gcge_reorth_tol d] [-gcge_scbgs_reorth_tol d]\n"
                "       [-gcge_max_reorth_time i] [-gcge_max_direct_orth_length i] [-gcge_cg_max_it i]\n"
                "       [-gcge_cg_rate d] [-gcge_cg_type 1|2] [-gcge_if_use_bcg 0|1] [-gcge_print_* i]\n"
-               "       [-gcge_bcg_precision 0|1] [-gcge_bcg_precond 0|1] [-gcge_lock_converged 0|1]\n");
+               "       [-gcge_bcg_precision 0|1] [-gcge_bcg_precond 0|1] [-gcge_lock_converged 0|1]\n"
+               "       [-gcge_rr_eig_full 0|1]\n");
         return 1;
     }
     return 0;

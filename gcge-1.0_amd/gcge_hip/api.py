@@ -22,7 +22,8 @@ _CLI = {"ev_max_it", "block_size", "given_init_evec", "ev_tol", "conv_type", "co
         "max_reorth_time", "max_direct_orth_length", "multi_tol", "multi_tol_for_lock", "if_use_bcg",
         "cg_max_it", "cg_rate", "cg_type", "print_eval", "print_part_time", "print_para", "print_result",
         "opt_rr_eig_partly", "x_orth_block_size", "w_orth_block_size", "if_use_cg", "print_cg_error",
-        "print_orth_zero", "scbgs_wself_max_reorth_time", "bcg_precision", "lock_converged", "bcg_precond"}
+        "print_orth_zero", "scbgs_wself_max_reorth_time", "bcg_precision", "lock_converged", "bcg_precond",
+        "rr_eig_full"}
 # members set directly after initialisation (read at solve time)
 _DIRECT = {"dirichlet_boundary", "stagnation_window", "stagnation_ratio", "stagnation_start"}
 
@@ -283,9 +284,9 @@ def synchronize():
 # fp32_tol, the iteration budgets -- the caller's ev_max_it -- and stagnation_ratio)
 MIXED_DEFAULTS = dict(fp32_tol=1e-7, fp32_max_it=0, fp64_max_it=0, stagnation_window=10, stagnation_ratio=0.7,
                       fp32_w_orth=1, refine_w_orth=1, refine_bcg_precision=1, stagnation_start=1e-3, cg_max_it=30,
-                      bcg_precond=1)
+                      bcg_precond=1, rr_eig_full=1)
 FP32_REFINE_DEFAULTS = dict(MIXED_DEFAULTS, fp32_w_orth=0, refine_w_orth=0, refine_bcg_precision=0, cg_max_it=0,
-                            bcg_precond=0)
+                            bcg_precond=0, rr_eig_full=0)
 
 
 def solve(A, B, nev, X0=None, precision="fp64", mixed=None, seed=None, keep_evec=True, **params):

@@ -65,7 +65,7 @@ class GCGE_PARA(C.Structure):
         ("solve_seconds", c_double),
         ("stagnation_window", c_int), ("stagnation_ratio", c_double), ("stopped_on_stagnation", c_int),
         ("bcg_precision", c_int), ("stagnation_start", c_double), ("lock_converged", c_int),
-        ("basis_eps", c_double), ("bcg_precond", c_int),
+        ("basis_eps", c_double), ("bcg_precond", c_int), ("rr_eig_full", c_int),
     ]
 
 
@@ -114,7 +114,7 @@ class GCGE_HIP_MIXED_OPTS(C.Structure):
     _fields_ = [("fp32_tol", c_double), ("fp32_max_it", c_int), ("fp64_max_it", c_int),
                 ("stagnation_window", c_int), ("stagnation_ratio", c_double), ("fp32_w_orth", c_int),
                 ("refine_w_orth", c_int), ("refine_bcg_precision", c_int), ("stagnation_start", c_double),
-                ("cg_max_it", c_int), ("bcg_precond", c_int)]
+                ("cg_max_it", c_int), ("bcg_precond", c_int), ("rr_eig_full", c_int)]
 
 
 class GCGE_HIP_MIXED_INFO(C.Structure):

@@ -91,6 +91,8 @@ typedef struct {
                                  (0: the caller's para->cg_max_it)                            */
     GCGE_INT bcg_precond;     /* 1: precondition the inner solves of both phases with the
                                  diagonal of A (Jacobi); 0: the caller's para->bcg_precond  */
+    GCGE_INT rr_eig_full;     /* 1: Rayleigh-Ritz pairs from the full dsyevd decomposition in
+                                 both phases (para->rr_eig_full); 0: the caller's setting     */
 } GCGE_HIP_MIXED_OPTS;
 
 typedef struct {
@@ -109,10 +111,11 @@ typedef struct {
  * applies to the refinement.  When given_init_evec is set, evec holds the
  * start block.  opts may be NULL: fp32_tol 1e-7, the caller's ev_max_it for
  * both phases, stagnation window 10 / ratio 0.7 / start 1e-3, CholQR in both
- * phases, FP32 inner solves in the refinement, and inner solves
- * preconditioned with the diagonal of A and capped at 30 CG steps.  In opts,
- * a zero fp32_tol, budget or stagnation_ratio selects the same default, and a
- * zero cg_max_it or bcg_precond keeps the caller's. */
+ * phases, FP32 inner solves in the refinement, inner solves preconditioned
+ * with the diagonal of A and capped at 30 CG steps, and Rayleigh-Ritz pairs
+ * from the full dsyevd decomposition.  In opts, a zero fp32_tol, budget or
+ * stagnation_ratio selects the same default, and a zero cg_max_it,
+ * bcg_precond or rr_eig_full keeps the caller's. */
 GCGE_INT GCGE_HIP_SOLVER_SolveMixed(GCGE_SOLVER *solver, const GCGE_HIP_MIXED_OPTS *opts,
                                     GCGE_HIP_MIXED_INFO *info);
 

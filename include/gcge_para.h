@@ -151,6 +151,13 @@ typedef struct GCGE_PARA_ {
      * The stopping rule stays ||r_k|| / ||r_0|| < cg_rate.  Needs the
      * BlockPCGResidualUpdate slot. */
     GCGE_INT bcg_precond;
+    /* input: 1 takes the Rayleigh-Ritz eigenpairs il..iu from the full
+     * decomposition of the subspace matrix (LAPACK dsyevd, divide and
+     * conquer: about 3x faster than dsyevx at 300 x 300, and orthonormal
+     * eigenvectors inside clusters of equal eigenvalues) instead of the
+     * DenseMatEigenSolver slot (dsyevx with range "I", the reference's
+     * choice, kept by default: iteration counts follow it). */
+    GCGE_INT rr_eig_full;
 } GCGE_PARA;
 
 void GCGE_PARA_Create(GCGE_PARA **para);
