@@ -13,11 +13,17 @@
 // FP64 phase: the FP32 eigenvectors are widened to FP64 and handed to the
 // same solver as the initial block (given_init_evec), which B-orthonormalizes
 // them and iterates to the caller's tolerance, optionally with FP32 inner
-// solves (refine_bcg_precision) and CholQR (refine_w_orth).
+// solves (refine_bcg_precision) and CholQR (refine_w_orth).  The iteration
+// locks a pair once it has converged and does not check it again, so a
+// locked pair can drift past the tolerance while the others converge; the
+// refinement therefore ends with a check of every pair and, while one is
+// above the tolerance, restarts from the current vectors (at most twice).
 #include <cfloat>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <vector>
 
 #include "gcge_app_hip.h"
 
@@ -25,6 +31,32 @@ namespace {
 double now_s()
 {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// Largest residual of the solver's nev eigenpairs under its convergence
+// criterion (the formulas of GCGE_CheckConvergence).
+double max_residual(GCGE_SOLVER *solver)
+{
+    GCGE_PARA *P = solver->para;
+    const int nev = P->nev;
+    const char *ct = P->conv_type;
+    const bool ns = std::strcmp(ct, "NS") == 0;
+    std::vector<GCGE_INT> idx(nev);
+    std::vector<double> rn(nev), xn(nev), bn(nev);
+    for (int i = 0; i < nev; i++) idx[i] = i;
+    solver->ops->MultiVecResidualNorms(solver->A, solver->B, solver->evec, idx.data(), nev, solver->eval, rn.data(),
+                                       xn.data(), ns ? bn.data() : nullptr, solver->ops);
+    double mx = 0.0;
+    for (int i = 0; i < nev; i++) {
+        const double l = std::fabs(solver->eval[i]);
+        double r;
+        if (std::strcmp(ct, "R") == 0) r = rn[i] / xn[i] / (1.0 > l ? 1.0 : l);
+        else if (std::strcmp(ct, "O") == 0) r = rn[i] / xn[i] / (P->conv_omega_norm + l);
+        else if (ns) r = rn[i] / (l * bn[i]);
+        else r = rn[i] / xn[i];
+        if (!(r <= mx)) mx = r;  // NaN propagates
+    }
+    return mx;
 }
 }  // namespace
 
@@ -128,20 +160,31 @@ extern "C" GCGE_INT GCGE_HIP_SOLVER_SolveMixed(GCGE_SOLVER *solver, const GCGE_H
     P->bcg_precond = o.bcg_precond;
     P->rr_eig_full = o.rr_eig_full;
     if (o.refine_w_orth) P->w_orth_type = (char *)"cholqr";
-    P->num_iter = 0;
-    P->num_unlock = nev;
-    P->sum_res = 0.0;
-    for (int i = 0; i < nev; i++) P->res[i] = 1.0;
-    // the workspace bookkeeping restarts as after GCGE_WORKSPACE_Setup
-    GCGE_WORKSPACE *ws = solver->workspace;
-    ws->dim_x = nev;
-    ws->dim_xp = 0;
-    ws->dim_xpw = nev;
-    ws->num_soft_locked_in_last_iter = 0;
-    ws->unconv_bs = (nev < P->block_size) ? nev : P->block_size;
-    for (int i = 0; i < ws->max_dim_x; i++) ws->unlock[i] = i;
-    GCGE_SOLVER_Solve(solver);
-    GCGE_HIP_Synchronize();
+    int iters = 0, bcg_iters = 0, bcg_calls = 0;
+    for (int pass = 0; pass < 3; pass++) {
+        P->num_iter = 0;
+        P->num_unlock = nev;
+        P->sum_res = 0.0;
+        for (int i = 0; i < nev; i++) P->res[i] = 1.0;
+        // the workspace bookkeeping restarts as after GCGE_WORKSPACE_Setup
+        GCGE_WORKSPACE *ws = solver->workspace;
+        ws->dim_x = nev;
+        ws->dim_xp = 0;
+        ws->dim_xpw = nev;
+        ws->num_soft_locked_in_last_iter = 0;
+        ws->unconv_bs = (nev < P->block_size) ? nev : P->block_size;
+        for (int i = 0; i < ws->max_dim_x; i++) ws->unlock[i] = i;
+        GCGE_SOLVER_Solve(solver);
+        GCGE_HIP_Synchronize();
+        iters += P->num_iter;
+        bcg_iters += P->bcg_iters_total;
+        bcg_calls += P->bcg_calls;
+        if (P->num_unlock != 0 || GCGE_HIP_GetLastError(nullptr, 0) != 0) break;  // budget spent, or an error
+        if (max_residual(solver) <= P->ev_tol) break;
+    }
+    P->num_iter = iters;  // the statistics of the passes added up
+    P->bcg_iters_total = bcg_iters;
+    P->bcg_calls = bcg_calls;
     P->given_init_evec = given;
     P->ev_max_it = maxit;
     P->bcg_precision = bcgp;
