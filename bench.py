@@ -87,8 +87,9 @@ ROCPROF_KERNEL = {
     "spmm_dot_f64": "k_spmm<double, true, ...>" + _ROWS.format("double", "true"),
     "spmm_dot_f32": "k_spmm<float, true, ...>" + _ROWS.format("float", "true"),
     "axpby": "k_axpby<...>", "dots": "k_dots<...>",
-    "cg_update_f64": "k_cg_resid<double>", "cg_update_f32": "k_cg_resid<float>",
-    "cg_search_f64": "k_cg_search<double>", "cg_search_f32": "k_cg_search<float>",
+    "cg_update_f64": "k_cg_resid<double> / k_pcg_resid<double>",
+    "cg_update_f32": "k_cg_resid<float> / k_pcg_resid<float>",
+    "cg_search_f64": "k_cg_search<double, ...>", "cg_search_f32": "k_cg_search<float, ...>",
     "gram_mfma_f64": "k_gram_mfma / k_gram_slab<double, ...>",
     "gram_mfma_f32": "k_gram_mfma / k_gram_slab<float, ...>",
     "gram_thin": "k_gram_thin<...>", "lincomb_mfma_f64": "k_lincomb_mfma<double, ...>",
