@@ -17,7 +17,8 @@ at --cg-max-it CG steps in both phases).  After the timed steps, FP64 solves
 from the same X0 are timed for the comparison: the drop-in with the
 reference's defaults ("fp64": unpreconditioned block CG of at most 20 steps),
 and FP64 with every algorithm setting of the mixed run -- CholQR, the same
-preconditioner and step cap -- so that only the precision differs
+preconditioner and step cap, the dsyevd Rayleigh-Ritz -- so that only the
+precision differs
 ("fp64_same_settings").  The device kernels are
 timed with HIP events on the backend stream over the timed region; the CPU
 baseline is the reference built from its own sources (oracle/_ref/ref_solve)
