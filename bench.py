@@ -14,7 +14,9 @@ orthonormalization, FP64 dense Rayleigh-Ritz and residual checks -- then FP64
 refinement sweeps of GCG from the FP32 eigenvectors with FP32 inner solves;
 inner solves preconditioned with the diagonal of A (--bcg-precond) and capped
 at --cg-max-it CG steps in both phases).  After the timed steps, FP64 solves
-from the same X0 are timed for the comparison: the drop-in with the
+from the same X0 are timed for the comparison (every solver is set up
+before its timed call and the timed calls run with the kernel profiler off;
+one extra profiled solve of the same work gives the per-kernel rates): the drop-in with the
 reference's defaults ("fp64": unpreconditioned block CG of at most 20 steps),
 and FP64 with every algorithm setting of the mixed run -- CholQR, the same
 preconditioner and step cap, the dsyevd Rayleigh-Ritz -- so that only the
@@ -174,7 +176,8 @@ def cpu_baseline(args, n_full, nnz_full, fp64_iters, nev):
     projected = per_iter * scale * max(1, fp64_iters)
     log(f"cpu baseline: {kind} {iters} iterations at n={A.shape[0]} in {secs:.2f}s "
         f"({time.perf_counter() - t0:.1f}s wall), projected {projected:.1f}s for the full solve")
-    return dict(value=nev / projected, unit="eigenpairs/s", cores=1, kind=kind,
+    model, nproc = host_cpu()
+    return dict(value=nev / projected, unit="eigenpairs/s", cores=1, kind=kind, host_cpu=model, host_nproc=nproc,
                 sample=(f"{iters} outer iterations of the reference solve (1 core, MKL 1 thread) on "
                         f"{args.workload} size {m} (n={A.shape[0]}, nnz={A.nnz}) with nev={nev}; "
                         f"time per iteration scaled by (nnz+2n) to the full problem and multiplied by "
@@ -197,6 +200,60 @@ def copy_probe():
     n = A.shape[0]
     print(json.dumps({"copy_probe": {"kernel": "k_copy<double, double>", "rows": n, "columns": k,
                                      "read_bytes": 8 * n * k, "write_bytes": 8 * n * k, "launches": 3}}))
+
+
+def spmm_probe(args):
+    """Three launches each of the FP64 and the FP32 SpMM with fused dots over
+    36 and 100 columns on the workload's matrix, for profiler runs
+    (rocprofv3 --kernel-trace / --pmc); prints the HIP-event rate of each."""
+    from gcge_hip import Matrix, MultiVec, Ops, capi
+    L = capi.load()
+    A, _ = make_problem(args)
+    mA = Matrix(A)
+    ops = Ops()
+    out = []
+    for prec, w in ((capi.GCGE_HIP_FP64, 8), (capi.GCGE_HIP_FP32, 4)):
+        for k in (36, 100):
+            V = MultiVec(mA, k, prec)
+            W = MultiVec(mA, k, prec)
+            ops.axpby(0.0, V, 0.0, V)
+            ops.matdot_dot(mA, V, W)
+            L.GCGE_HIP_Synchronize()
+            L.GCGE_HIP_ResetKernelStats()
+            L.GCGE_HIP_SetProfiling(1)
+            for _ in range(3):
+                ops.matdot_dot(mA, V, W)
+            L.GCGE_HIP_Synchronize()
+            L.GCGE_HIP_SetProfiling(0)
+            st = capi.kernel_stats()
+            cls = "spmm_dot_f64" if prec == capi.GCGE_HIP_FP64 else "spmm_dot_f32"
+            s = st[cls]
+            alg = (w + 4) * A.nnz + 4 * (A.shape[0] + 1) + 2 * k * A.shape[0] * w
+            ms = s["ms"] / max(1, s["count"])
+            out.append(dict(kernel=cls, columns=k, avg_ms=round(ms, 4), alg_bytes=alg,
+                            gbs=round(alg / (ms * 1e-3) / 1e9, 1)))
+            log(f"spmm probe {cls} k={k}: {ms:.4f} ms, {alg / (ms * 1e-3) / 1e9:.0f} GB/s")
+            V.close()
+            W.close()
+    print(json.dumps({"spmm_probe": out, "workload": f"{args.workload} {args.size}"}))
+
+
+def host_cpu():
+    """CPU model name and the cores this process may use (the box's share)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        nproc = os.cpu_count()
+    return model, nproc
 
 
 def main():
@@ -229,16 +286,24 @@ def main():
     ap.add_argument("--no-fp64-same", action="store_true",
                     help="skip the FP64 solve with the mixed path's algorithm settings (CholQR W "
                          "orthonormalization, the same inner CG step cap)")
+    ap.add_argument("--no-profiled-step", action="store_true",
+                    help="skip the extra profiled solve that gives the per-kernel rates")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-sample-size", type=int, default=24)
+    ap.add_argument("--cpu-sample-size", type=int, default=48)
     ap.add_argument("--cpu-sample-iters", type=int, default=3)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run k_copy<double, double> over 16 columns of n = 4,096,000 rows three times "
                          "(1 GiB moved per launch), the reference of known size for rocprofv3 --pmc byte counters")
+    ap.add_argument("--spmm-probe", action="store_true",
+                    help="only time the FP64 and FP32 SpMM (fused dots, 36 and 100 columns) on the workload's "
+                         "matrix, three launches each, for profiler runs")
     args = ap.parse_args()
 
     if args.copy_probe:
         copy_probe()
+        return
+    if args.spmm_probe:
+        spmm_probe(args)
         return
 
     if args.gpus != 1:
@@ -247,7 +312,7 @@ def main():
         log("bench.py: only --gpus 1 is supported (one process drives one GPU)")
         sys.exit(2)
 
-    from gcge_hip import Matrix, MultiVec, capi, solve
+    from gcge_hip import Matrix, MultiVec, Solver, capi
     from gcge_hip import problems as P
     L = capi.load()
 
@@ -268,68 +333,91 @@ def main():
                  stagnation_window=args.stagnation_window, stagnation_ratio=0.7,
                  stagnation_start=args.stagnation_start, cg_max_it=args.cg_max_it, bcg_precond=args.bcg_precond)
 
-    def step(keep=False):
-        return solve(mA, mB, nev, X0=X0, precision=args.mode, mixed=mixed, keep_evec=keep, **common)
+    def ns_of(solver, res):
+        """NS residual of every pair, recomputed in FP64 on the device, and
+        the strict convergence verdict: no unconverged pair reported, every
+        residual finite and within the tolerance."""
+        rn, xn, bn = solver.residual_norms()
+        ev = np.asarray(res["eval"])
+        ns = rn / (np.abs(ev) * bn)
+        ok = res["num_unlock"] == 0 and bool(np.all(np.isfinite(ns))) and float(ns.max()) <= args.tol * (1 + 1e-12)
+        return ns, ok
 
+    def timed(solver, what):
+        L.GCGE_HIP_Synchronize()
+        t0 = time.perf_counter()
+        r = solver.solve(X0, keep_evec=False)
+        L.GCGE_HIP_Synchronize()
+        dt = time.perf_counter() - t0
+        log(f"{what}: {dt:.3f}s  {r['num_iter']} it, unconverged {r['num_unlock']} {r['info']}")
+        return r, dt
+
+    def profiled(solver, what):
+        """One more solve of the same work with the per-launch HIP-event
+        profiler on: kernel classes, launch counts and device time."""
+        L.GCGE_HIP_Synchronize()
+        L.GCGE_HIP_SetProfiling(1)
+        before = capi.kernel_stats()
+        t0 = time.perf_counter()
+        r = solver.solve(X0, keep_evec=False)
+        L.GCGE_HIP_Synchronize()
+        dt = time.perf_counter() - t0
+        stats = kernel_delta(capi.kernel_stats(), before)
+        L.GCGE_HIP_SetProfiling(0)
+        kms = sum(v["ms"] for v in stats.values())
+        log(f"{what} (profiled): {dt:.3f}s, kernels {kms / 1e3:.3f}s")
+        return r, dt, stats, kms
+
+    # ---- the timed steps: one solver set up once (workspace allocated
+    # outside the timed region), solves from the same X0, profiler off
+    S = Solver(mA, mB, nev, precision=args.mode, mixed=mixed, **common)
+    warm = []
     for i in range(args.warmup):
-        r = step()
-        log(f"warmup {i}: {r['wall']:.2f}s  {r['num_iter']} it, unconverged {r['num_unlock']} {r['info']}")
-    L.GCGE_HIP_SetProfiling(1)
-    before = capi.kernel_stats()
+        warm.append(timed(S, f"warmup {i}")[1])
     L.GCGE_HIP_Synchronize()
     t0 = time.perf_counter()
-    results = []
+    step_s = []
+    last = None
     for i in range(args.steps):
-        results.append(step(keep=(i == args.steps - 1)))
-        log(f"step {i}: {results[-1]['wall']:.2f}s")
+        ts = time.perf_counter()
+        last = S.solve(X0, keep_evec=False)
+        L.GCGE_HIP_Synchronize()
+        step_s.append(time.perf_counter() - ts)
+        log(f"step {i}: {step_s[-1]:.3f}s  {last['num_iter']} it {last['info']}")
     L.GCGE_HIP_Synchronize()
     t1 = time.perf_counter()
-    after = capi.kernel_stats()
-    L.GCGE_HIP_SetProfiling(0)
-    mixed_stats = kernel_delta(after, before)
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     value = nev / (ms_per_step * 1e-3)
-    last = results[-1]
-    converged = last["num_unlock"] == 0
-
-    # residual check of the mixed result on the device, FP64 (NS criterion)
-    from gcge_hip import Ops
-    ops = Ops()
-    xv = MultiVec(mA, nev, data=last["evec"])
-    rn, xn, bn = ops.residual_norms(mA, mB, xv, np.arange(nev), last["eval"])
-    ns = rn / (np.abs(last["eval"]) * bn)
-    xv.close()
-    ops.close()
+    ns, converged = ns_of(S, last)
+    mixed_eval = np.asarray(last["eval"])
+    prof = None
+    if not args.no_profiled_step:
+        rp, dtp, stats, kms = profiled(S, "profiled step")
+        prof = dict(seconds=dtp, stats=stats, kernel_s=kms / 1e3, iters=rp["num_iter"])
+    S.close()
 
     fp64 = None
     if not args.no_fp64:
-        # timed like a step: the whole solve call, solver set-up included
-        L.GCGE_HIP_SetProfiling(1)
-        b64 = capi.kernel_stats()
-        L.GCGE_HIP_Synchronize()
-        t64 = time.perf_counter()
-        r64 = solve(mA, mB, nev, X0=X0, precision="fp64", keep_evec=False, **common)
-        L.GCGE_HIP_Synchronize()
-        t64 = time.perf_counter() - t64
-        a64 = capi.kernel_stats()
-        L.GCGE_HIP_SetProfiling(0)
-        fp64 = dict(seconds=t64, iters=r64["num_iter"], unconverged=r64["num_unlock"],
-                    eval=r64["eval"], stats=kernel_delta(a64, b64))
-        log(f"fp64 (reference defaults): {t64:.2f}s ({r64['wall']:.2f}s in the solve), {r64['num_iter']} it, "
-            f"unconverged {r64['num_unlock']}")
+        # the FP64 drop-in (reference defaults), set up outside the timed call
+        S64 = Solver(mA, mB, nev, precision="fp64", **common)
+        r64, t64 = timed(S64, "fp64 (reference defaults)")
+        ns64, ok64 = ns_of(S64, r64)
+        fp64 = dict(seconds=t64, iters=r64["num_iter"], converged=ok64, eval=np.asarray(r64["eval"]),
+                    max_ns=float(np.max(ns64)))
+        S64.close()
     fp64s = None
     if not args.no_fp64_same and args.mode == "mixed":
-        L.GCGE_HIP_Synchronize()
-        t64s = time.perf_counter()
-        r64s = solve(mA, mB, nev, X0=X0, precision="fp64", keep_evec=False, w_orth_type="cholqr",
-                     cg_max_it=args.cg_max_it, bcg_precond=args.bcg_precond, rr_eig_full=1, **common)
-        L.GCGE_HIP_Synchronize()
-        t64s = time.perf_counter() - t64s
-        fp64s = dict(seconds=t64s, iters=r64s["num_iter"], unconverged=r64s["num_unlock"],
-                     eval=r64s["eval"])
-        log(f"fp64 with the mixed path's settings (CholQR, cg_max_it {args.cg_max_it}, bcg_precond "
-            f"{args.bcg_precond}, dsyevd Rayleigh-Ritz): {t64s:.2f}s, "
-            f"{r64s['num_iter']} it, unconverged {r64s['num_unlock']}")
+        # FP64 with every algorithm setting of the mixed run: timed like a step
+        S64s = Solver(mA, mB, nev, precision="fp64", w_orth_type="cholqr", cg_max_it=args.cg_max_it,
+                      bcg_precond=args.bcg_precond, rr_eig_full=1, **common)
+        r64s, t64s = timed(S64s, "fp64 with the mixed path's settings")
+        ns64s, ok64s = ns_of(S64s, r64s)
+        fp64s = dict(seconds=t64s, iters=r64s["num_iter"], converged=ok64s, eval=np.asarray(r64s["eval"]),
+                     max_ns=float(np.max(ns64s)))
+        if not args.no_profiled_step:
+            _, dtp, st64, kms64 = profiled(S64s, "fp64 same settings")
+            fp64s.update(stats=st64, profiled_seconds=dtp, kernel_s=kms64 / 1e3)
+        S64s.close()
 
     cpu = None
     if not args.no_cpu:
@@ -340,7 +428,8 @@ def main():
             log(f"cpu baseline failed: {e}")
 
     wl = f"{args.workload} {args.size}"
-    dom, roof = roofline(mixed_stats, "fp64" if args.mode == "fp64" else "mixed", wl)
+    stats = prof["stats"] if prof else {}
+    dom, roof = roofline(stats, "fp64" if args.mode == "fp64" else "mixed", wl)
     line = {
         "metric": "Eigenpairs/sec to 1e-8 rel. residual, mixed vs FP64 GCGE, 1 MI355X",
         "value": round(value, 4),
@@ -364,37 +453,49 @@ def main():
                    "stagnation": [args.stagnation_window, 0.7, args.stagnation_start]},
         "converged": bool(converged),
         "max_ns_residual": float(np.max(ns)),
+        "step_seconds": [round(x, 3) for x in step_s],
+        "warmup_seconds": [round(x, 3) for x in warm],
+        "timing": "timed steps with the kernel profiler off; per-kernel rates from one extra profiled step",
         "mixed": dict(mode=args.mode, iters=last["num_iter"], bcg_sweeps=last["bcg_iters_total"],
                       **{k: (round(v, 3) if isinstance(v, float) else v) for k, v in last["info"].items()}),
         "roofline_kernel": dom,
         "roofline": roof,
         "kernels_mixed": {k: dict(count=v["count"], ms=round(v["ms"], 2),
                                   gbs=round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else None)
-                          for k, v in mixed_stats.items() if v["count"]},
+                          for k, v in stats.items() if v["count"]},
         "cpu_baseline": cpu,
     }
+    if prof:
+        line["profiled_step"] = dict(seconds=round(prof["seconds"], 3), kernel_seconds=round(prof["kernel_s"], 3),
+                                     outside_kernels_seconds=round(prof["seconds"] - prof["kernel_s"], 3))
     if fp64:
-        d64, roof64 = roofline(fp64["stats"], "fp64", wl)
         line["fp64"] = dict(eigenpairs_per_s=round(nev / fp64["seconds"], 4), seconds=round(fp64["seconds"], 3),
-                            iters=fp64["iters"], converged=fp64["unconverged"] == 0,
-                            roofline_kernel=d64, roofline=roof64,
-                            kernels={k: dict(count=v["count"], ms=round(v["ms"], 2),
-                                             gbs=round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1)
-                                             if v["ms"] > 0 else None)
-                                     for k, v in fp64["stats"].items() if v["count"]})
+                            iters=fp64["iters"], converged=fp64["converged"], max_ns_residual=fp64["max_ns"])
         line["mixed_vs_fp64"] = round(fp64["seconds"] / (ms_per_step * 1e-3), 3)
-        line["max_rel_eig_diff_mixed_vs_fp64"] = float(np.max(np.abs(last["eval"] - fp64["eval"]) /
-                                                           np.abs(fp64["eval"])))
     if fp64s:
         # the precision-only comparison: FP64 with every algorithm setting of
         # the mixed path (CholQR, inner CG step cap and preconditioner, dense
-        # Rayleigh-Ritz solver)
+        # Rayleigh-Ritz solver), timed the same way as the steps
         line["fp64_same_settings"] = dict(eigenpairs_per_s=round(nev / fp64s["seconds"], 4),
                                           seconds=round(fp64s["seconds"], 3), iters=fp64s["iters"],
-                                          converged=fp64s["unconverged"] == 0, w_orth_type="cholqr",
-                                          cg_max_it=args.cg_max_it, bcg_precond=args.bcg_precond,
-                                          rr_eig="dsyevd")
+                                          converged=fp64s["converged"], max_ns_residual=fp64s["max_ns"],
+                                          w_orth_type="cholqr", cg_max_it=args.cg_max_it,
+                                          bcg_precond=args.bcg_precond, rr_eig="dsyevd")
+        if "stats" in fp64s:
+            d64, roof64 = roofline(fp64s["stats"], "fp64", wl)
+            line["fp64_same_settings"].update(
+                roofline_kernel=d64, roofline=roof64,
+                kernels={k: dict(count=v["count"], ms=round(v["ms"], 2),
+                                 gbs=round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else None)
+                         for k, v in fp64s["stats"].items() if v["count"]})
         line["mixed_vs_fp64_same_settings"] = round(fp64s["seconds"] / (ms_per_step * 1e-3), 3)
+    ref = fp64s or fp64
+    if ref:
+        # the error of every mixed eigenvalue against the FP64 solve
+        rel = np.abs(mixed_eval - ref["eval"]) / np.abs(ref["eval"])
+        line["max_rel_eig_diff_mixed_vs_fp64"] = float(rel.max())
+        line["rel_eig_diff_per_pair_vs_fp64"] = [float("%.2e" % x) for x in rel]
+        line["rel_eig_diff_reference"] = "fp64_same_settings" if fp64s else "fp64"
     print(json.dumps(line), flush=True)
 
 

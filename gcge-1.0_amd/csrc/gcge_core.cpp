@@ -27,6 +27,12 @@
 
 #include "gcge.h"
 
+// error latch of the library (hip_backend.cpp): read by GCGE_HIP_GetLastError
+namespace gh {
+void latch(int code, const char *fmt, ...);
+bool failed();
+}  // namespace gh
+
 namespace {
 
 double now_s()
@@ -134,6 +140,14 @@ void orth_gs(void **V, int start, int *end, void *B, GCGE_OPS *ops, GCGE_PARA *p
                 ratio = vout / vin;
                 reorth_count++;
             } while (ratio < op->reorth_tol && reorth_count < op->max_reorth_time && vout > op->orth_zero_tol);
+        }
+        if (!std::isfinite(vin)) {
+            // a non-finite B-norm (NaN or Inf in the vector or in B) would pass
+            // as a zero vector and be deflated; a NaN vout from a negative norm
+            // estimate is the reference's zero-vector signal and is kept
+            gh::latch(191, "Orthonormalization: B-norm of column %d is %g", current, vin);
+            *end = current;
+            return;
         }
         if (vout > op->orth_zero_tol) {
             ops->MultiVecAxpbyColumn(0.0, V, current, 1 / vout, V, current, ops);
@@ -278,6 +292,239 @@ void stable_multi_orth(void **V, int start, int *end, void *B, GCGE_OPS *ops, GC
         if (iter >= para->orth_para->max_reorth_time) break;
     }
     multi_orth(V, start, end, B, ops, para, ws);
+}
+
+// ---- the other W orthonormalizations of the reference (w_orth_type "bgs",
+// "cbgs", "scbgs"), issued with the same per-step structure on block slots.
+
+// Y = B X for columns [xs, xe) of X into columns [ys, ...) of Y; a copy
+// when B is NULL (the reference's VecAxpby(1.0, x, 0.0, y)).
+void b_times(void *B, void **X, int xs, int xe, void **Y, int ys, GCGE_OPS *ops)
+{
+    if (xe <= xs) return;
+    GCGE_INT ms[2], me[2];
+    set2(ms, xs, ys);
+    set2(me, xe, ys + (xe - xs));
+    if (B) ops->MatDotMultiVec(B, X, Y, ms, me, ops);
+    else ops->MultiVecCopy(X, Y, ms, me, ops);
+}
+
+// V[j] -= d[j - s] V[cur] for j in [s, e): one rank-1 combination.
+void minus_rank1(void **V, int cur, int s, int e, const double *d, double scale, GCGE_OPS *ops,
+                 std::vector<double> &coef)
+{
+    if (e <= s) return;
+    GCGE_INT ms[2], me[2];
+    coef.resize(e - s);
+    for (int j = s; j < e; j++) coef[j - s] = -d[j - s] / scale;
+    set2(ms, cur, s);
+    set2(me, cur + 1, e);
+    ops->MultiVecLinearCombAxpby(V, V, ms, me, coef.data(), 1, 1.0, 1.0, ops);
+}
+
+// Modified Gram-Schmidt of V[start:end] against itself in the B inner
+// product, column by column, with zero columns swapped to the end and
+// dropped: the W-self loop of GCGE_BOrthonormalization
+// (gcge_orthogonal.c:459-526) and of GCGE_SCBOrth_Self (:1196-1279).
+void w_self_mgs(void **V, int start, int *end, void *B, GCGE_OPS *ops, GCGE_PARA *para, GCGE_WORKSPACE *ws,
+                const char *who)
+{
+    GCGE_ORTH_PARA *op = para->orth_para;
+    double *d = ws->subspace_dtmp;
+    GCGE_INT ms[2], me[2];
+    std::vector<double> coef;
+    for (int cur = start; cur < *end; cur++) {
+        b_times(B, V, cur, cur + 1, ws->CG_p, 0, ops);
+        set2(ms, cur, 0);
+        set2(me, *end, 1);
+        ops->MultiVecInnerProd(V, ws->CG_p, d, (char *)"ns", ms, me, *end - cur, ops);
+        const double norm_value = std::sqrt(d[0]);
+        if (norm_value > op->orth_zero_tol) {
+            ops->MultiVecAxpbyColumn(0.0, V, cur, 1.0 / norm_value, V, cur, ops);
+            minus_rank1(V, cur, cur + 1, *end, d + 1, norm_value, ops, coef);
+        } else {
+            set2(ms, cur, *end - 1);
+            set2(me, cur + 1, *end);
+            ops->MultiVecSwap(V, V, ms, me, ops);
+            (*end)--;
+            cur--;
+            if (op->print_orth_zero == 1)
+                printf("In %s, there is a zero vector!, current = %d, start = %d, end = %d\n", who, cur, start, *end);
+        }
+    }
+}
+
+// GCGE_BOrthonormalization, gcge_orthogonal.c:352-528: evec = B V[0:nev]
+// and V_tmp = B V[nev:dim_xp] once; then twice: W -= v_i (v_i^T B W) for
+// each X and P column in turn, and the W-self MGS.
+void orth_bgs(void **V, int start, int *end, void *B, GCGE_OPS *ops, GCGE_PARA *para, GCGE_WORKSPACE *ws)
+{
+    const int nev = para->nev, dim_xp = ws->dim_xp;
+    double *d = ws->subspace_dtmp;
+    GCGE_INT ms[2], me[2];
+    std::vector<double> coef;
+    b_times(B, V, 0, nev, ws->evec, 0, ops);
+    b_times(B, V, nev, dim_xp, ws->V_tmp, 0, ops);
+    for (int pass = 0; pass < 2; pass++) {
+        const int wl = *end - start;
+        for (int cur = 0; cur < dim_xp; cur++) {
+            void **BV = cur < nev ? ws->evec : ws->V_tmp;
+            const int bc = cur < nev ? cur : cur - nev;
+            set2(ms, start, bc);
+            set2(me, *end, bc + 1);
+            ops->MultiVecInnerProd(V, BV, d, (char *)"ns", ms, me, wl, ops);
+            minus_rank1(V, cur, start, *end, d, 1.0, ops, coef);
+        }
+        w_self_mgs(V, start, end, B, ops, para, ws, "Orthonormalization");
+    }
+}
+
+// W -= Vp (Vp^T B W) with Vp = V[p0:p1] and W = V[start:end]: V_tmp = B W,
+// d = Vp^T V_tmp (leading dimension lda), V_tmp = Vp d, W -= V_tmp
+// (gcge_orthogonal.c:822-855).  Returns sqrt of the sum of the squares of
+// d[0:nsum] (the reference's value_inner, read with the same extent).
+double project_out(void **V, int p0, int p1, int start, int end, int lda, int nsum, void *B, GCGE_OPS *ops,
+                   GCGE_WORKSPACE *ws)
+{
+    const int wl = end - start;
+    double *d = ws->subspace_dtmp;
+    GCGE_INT ms[2], me[2];
+    if (B) {
+        set2(ms, start, 0);
+        set2(me, end, wl);
+        ops->MatDotMultiVec(B, V, ws->V_tmp, ms, me, ops);
+    } else {
+        set2(ms, start, 0);
+        set2(me, end, wl);
+        ops->MultiVecAxpby(1.0, V, 0.0, ws->V_tmp, ms, me, ops);
+    }
+    set2(ms, p0, 0);
+    set2(me, p1, wl);
+    ops->MultiVecInnerProd(V, ws->V_tmp, d, (char *)"ns", ms, me, lda, ops);
+    double value_inner = 0.0;
+    for (int i = 0; i < nsum; i++) value_inner += d[i] * d[i];
+    ops->MultiVecLinearComb(V, ws->V_tmp, ms, me, d, lda, nullptr, -1, ops);
+    set2(ms, 0, start);
+    set2(me, wl, end);
+    ops->MultiVecAxpby(-1.0, ws->V_tmp, 1.0, V, ms, me, ops);
+    return std::sqrt(value_inner);
+}
+
+// GCGE_CBOrthonormalization, gcge_orthogonal.c:755-963: two block
+// projections against V[0:start], then W = W Y Theta^{-1/2} from the
+// eigenpairs of W^T B W (dsyevx), dropping eigenvalues below orth_zero_tol.
+void orth_cbgs(void **V, int start, int *end, void *B, GCGE_OPS *ops, GCGE_PARA *para, GCGE_WORKSPACE *ws)
+{
+    GCGE_ORTH_PARA *op = para->orth_para;
+    double *d = ws->subspace_dtmp;
+    GCGE_INT ms[2], me[2];
+    int wl = *end - start;
+    for (int pass = 0; pass < 2; pass++) {
+        wl = *end - start;
+        project_out(V, 0, start, start, *end, start, 0, B, ops, ws);
+    }
+    // M = W^T B W
+    b_times(B, V, start, *end, ws->V_tmp, 0, ops);
+    set2(ms, start, 0);
+    set2(me, *end, wl);
+    ops->MultiVecInnerProd(V, ws->V_tmp, d, (char *)"ns", ms, me, wl, ops);
+    bool nonzero = false;
+    for (int j = 0; j < wl * wl; j++)
+        if (d[j] > op->orth_zero_tol) {
+            nonzero = true;
+            break;
+        }
+    if (!nonzero) {
+        printf("W all zero!\n");
+        *end = start;
+        return;
+    }
+    // the reference's buffers: eigenvectors and eigenvalues in subspace_evec,
+    // LAPACK work behind the matrix in subspace_dtmp, integer work in subspace_itmp
+    double *tmp_evec = ws->subspace_evec, *tmp_eval = tmp_evec + (size_t)wl * wl, *dwork = d + (size_t)wl * wl;
+    int *itmp = ws->subspace_itmp, *isuppz = itmp + 10 * wl, *ifail = itmp + 5 * wl;
+    double vl = 0.0, vu = 0.0, abstol = 0.0;
+    int il = 1, iu = wl, m = wl, lwork = 8 * wl, liwork = 5 * wl, info = 0;
+    ops->DenseMatEigenSolver((char *)"V", (char *)"I", (char *)"U", &wl, d, &wl, &vl, &vu, &il, &iu, &abstol, &m,
+                             tmp_eval, tmp_evec, &wl, isuppz, dwork, &lwork, itmp, &liwork, ifail, &info);
+    int nz_start = 0;
+    for (int j = 0; j < wl; j++) {
+        if (tmp_eval[j] < op->orth_zero_tol) nz_start = j + 1;
+        else ops->ArrayScale(1.0 / std::sqrt(tmp_eval[j]), tmp_evec + (size_t)j * wl, wl);
+    }
+    if (nz_start) {
+        printf("tmp_eval_nonzero_start: %d\n", nz_start);
+        for (int j = 0; j < wl; j++) printf("tmp_eval[%d]: %e\n", j, tmp_eval[j]);
+    }
+    const int wl_new = wl - nz_start;
+    // V_tmp = W Y[:, nz_start:].  The reference passes the new column count
+    // as the leading dimension (gcge_orthogonal.c:938-945), which reads other
+    // coefficients once a column has been dropped (nz_start > 0); here Y is
+    // read with its own leading dimension wl.  Without dropped columns the two
+    // agree.
+    set2(ms, start, 0);
+    set2(me, *end, wl_new);
+    ops->MultiVecLinearComb(V, ws->V_tmp, ms, me, tmp_evec + (size_t)nz_start * wl, wl, nullptr, -1, ops);
+    *end = start + wl_new;
+    set2(ms, start, 0);
+    set2(me, *end, wl_new);
+    ops->MultiVecSwap(V, ws->V_tmp, ms, me, ops);
+}
+
+// GCGE_SCBOrthonormalization, gcge_orthogonal.c:967-1113: block projections
+// of W against V[0:start] until their size is below scbgs_reorth_tol (at most
+// max_reorth_time), then W in blocks of w_orth_block_size columns: each block
+// is projected against the W columns before it (GCGE_SCBOrth_Minus,
+// :1116-1193) and orthonormalized by MGS scbgs_wself_max_reorth_time times
+// (GCGE_SCBOrth_Self, :1196-1279); a block's dropped columns are swapped to
+// the end of W.
+void orth_scbgs(void **V, int start, int *end, void *B, GCGE_OPS *ops, GCGE_PARA *para, GCGE_WORKSPACE *ws)
+{
+    GCGE_ORTH_PARA *op = para->orth_para;
+    const double tol = op->scbgs_reorth_tol;
+    GCGE_INT ms[2], me[2];
+    {
+        const int wl = *end - start;  // fixed over the passes, as in the reference
+        double value_inner = 1.0;
+        int iter = 0;
+        while (value_inner > tol) {
+            iter++;
+            value_inner = project_out(V, 0, start, start, start + wl, start, start * wl, B, ops, ws);
+            if (iter >= op->max_reorth_time) break;
+        }
+    }
+    auto self = [&](int ws0, int *we) {
+        for (int it = 0; it < op->scbgs_wself_max_reorth_time; it++)
+            w_self_mgs(V, ws0, we, B, ops, para, ws, "Orthonormalization");
+    };
+    auto minus = [&](int ws0, int we) {
+        // the sum runs over (ws0 - dim_xp) * wl entries of a Gram stored with
+        // leading dimension ws0, as the reference reads it (:1149-1155)
+        const int wl = we - ws0, dim_xp = ws->dim_xp;
+        double value_inner = 1.0;
+        int iter = 0;
+        while (value_inner > tol) {
+            iter++;
+            value_inner = project_out(V, dim_xp, ws0, ws0, we, ws0, (ws0 - dim_xp) * wl, B, ops, ws);
+            if (iter >= op->max_reorth_time) break;
+        }
+    };
+    const int step = std::min(*end - start, op->w_orth_block_size);
+    int w_start = start, w_end = std::min(w_start + step, *end);
+    self(w_start, &w_end);
+    while (w_end < *end) {
+        w_start = w_end;
+        w_end = std::min(w_start + step, *end);
+        const int old_w_end = w_end;
+        minus(w_start, w_end);
+        self(w_start, &w_end);
+        if (w_end < old_w_end) {
+            *end -= old_w_end - w_end;
+            set2(ms, w_end, *end);
+            set2(me, old_w_end, *end + old_w_end - w_end);
+            ops->MultiVecSwap(V, V, ms, me, ops);
+        }
+    }
 }
 
 }  // namespace
@@ -860,6 +1107,14 @@ void rr_eigenpairs(double *subspace_matrix, double *eval, double *subspace_evec,
     for (int i = 0; i < ws->dim_xpw - ws->dim_xp; i++)
         std::memset(subspace_matrix + (size_t)(ws->dim_xp + i) * ldm, 0, (size_t)rr_eigen_start * sizeof(double));
     std::memcpy(temp, subspace_matrix, (size_t)ldm * ldm * sizeof(double));
+    for (size_t t = 0; t < (size_t)ldm * ldm; t++)
+        if (!std::isfinite(temp[t])) {
+            // NaN or Inf from the large-scale products: no LAPACK call on it
+            gh::latch(192, "Rayleigh-Ritz: subspace matrix entry (%d, %d) is %g", (int)(t % ldm), (int)(t / ldm),
+                      temp[t]);
+            for (int i = rr_eigen_start; i < ldm; i++) eval[i] = temp[t];
+            return;
+        }
     int nrows = ldm - rr_eigen_start, ncols = ldm;
     il -= rr_eigen_start;
     iu -= rr_eigen_start;
@@ -886,7 +1141,9 @@ void rr_eigenpairs(double *subspace_matrix, double *eval, double *subspace_evec,
 // the reference visits are computed in batches: pairs up to position
 // max_conv_idx + 9 of unlock[] are always visited (the early exit needs
 // j > max_conv_idx + 8), so each batch extends exactly to that horizon and no
-// residual is computed that the reference would not compute.
+// residual is computed that the reference would not compute.  A residual
+// that is NaN or infinite leaves its pair unconverged and latches error 190
+// (the reference counts a NaN as converged); GCGE_Solve then stops.
 extern "C" void GCGE_CheckConvergence(void *A, void *B, GCGE_DOUBLE *eval, void **evec, GCGE_OPS *ops,
                                       GCGE_PARA *para, GCGE_WORKSPACE *ws)
 {
@@ -924,11 +1181,15 @@ extern "C" void GCGE_CheckConvergence(void *A, void *B, GCGE_DOUBLE *eval, void 
                 residual = res_norm / evec_norm;
             para->res[i] = residual;
             sum_res += residual;
+            const bool finite = std::isfinite(residual);
+            if (!finite)
+                gh::latch(190, "GCGE_CheckConvergence: residual of eigenpair %d is %g (eigenvalue %g)", i, residual,
+                          eval[i]);
             if (j == 0) {
                 max_res = residual;
                 min_res = residual;
             } else {
-                if (residual > max_res) {
+                if (!(residual <= max_res) && !std::isnan(max_res)) {  // a NaN residual becomes the maximum
                     max_res = residual;
                     max_ind = i;
                 }
@@ -937,7 +1198,9 @@ extern "C" void GCGE_CheckConvergence(void *A, void *B, GCGE_DOUBLE *eval, void 
                     min_ind = i;
                 }
             }
-            if (residual > para->ev_tol) {
+            // the reference tests residual > ev_tol (gcge_eigsol.c:630), which
+            // passes a NaN residual as converged; here it stays unconverged
+            if (!(residual <= para->ev_tol)) {
                 unlock[unconv_bs] = i;
                 unconv_bs += 1;
                 if (j > max_conv_idx + 8) {
@@ -1039,12 +1302,17 @@ extern "C" void GCGE_Solve(void *A, void *B, GCGE_DOUBLE *eval, void **evec, GCG
                 ws->dim_x = nev;
                 orth_gs(evec, num_init_evec, &ws->dim_x, Orth_mat, ops, para, ws->V_tmp, ws->subspace_dtmp);
                 num_init_evec = ws->dim_x;
-            } while (num_init_evec < nev);
+            } while (num_init_evec < nev && !gh::failed());
         } else {
             orth_gs(evec, 0, &ws->dim_x, Orth_mat, ops, para, ws->V_tmp, ws->subspace_dtmp);
         }
     }
     GCGE_PrintParaInfo(para);
+    if (gh::failed()) {
+        // a latched error (e.g. a non-finite norm of the initial block): no iteration
+        para->solve_seconds = now_s() - t_start;
+        return;
+    }
 
     set2(ms, 0, 0);
     set2(me, nev, nev);
@@ -1116,7 +1384,7 @@ extern "C" void GCGE_Solve(void *A, void *B, GCGE_DOUBLE *eval, void **evec, GCG
         snap_pair_res.assign(para->res, para->res + nev);
         snap_res = para->max_res;
     };
-    while (para->num_unlock > 0 && para->num_iter < ev_max_it) {
+    while (para->num_unlock > 0 && para->num_iter < ev_max_it && !gh::failed()) {
         peak_res = std::max(peak_res, para->max_res);
         if (para->stagnation_window > 0 && para->stagnation_start > 0.0 &&
             !(para->max_res <= para->stagnation_start * peak_res) && !watching) {
@@ -1153,11 +1421,18 @@ extern "C" void GCGE_Solve(void *A, void *B, GCGE_DOUBLE *eval, void **evec, GCG
         ws->dim_xpw = ws->dim_xp + ws->unconv_bs;
         {
             GCGE_TIMER(w_orth_time, para);
-            if (std::strcmp(para->w_orth_type, "multi") == 0)
+            const char *wt = para->w_orth_type;
+            if (std::strcmp(wt, "multi") == 0)
                 stable_multi_orth(V, ws->dim_xp, &ws->dim_xpw, Orth_mat, ops, para, ws);
-            else if (std::strcmp(para->w_orth_type, "cholqr") == 0)
+            else if (std::strcmp(wt, "cholqr") == 0)
                 cholqr_orth(V, ws->dim_xp, &ws->dim_xpw, Orth_mat, ops, para, ws);
-            else
+            else if (std::strcmp(wt, "bgs") == 0)
+                orth_bgs(V, ws->dim_xp, &ws->dim_xpw, Orth_mat, ops, para, ws);
+            else if (std::strcmp(wt, "cbgs") == 0)
+                orth_cbgs(V, ws->dim_xp, &ws->dim_xpw, Orth_mat, ops, para, ws);
+            else if (std::strcmp(wt, "scbgs") == 0)
+                orth_scbgs(V, ws->dim_xp, &ws->dim_xpw, Orth_mat, ops, para, ws);
+            else  // "gs", and any other name, as in the reference (gcge_eigsol.c:468-472)
                 orth_gs(V, ws->dim_xp, &ws->dim_xpw, Orth_mat, ops, para, ws->V_tmp, ws->subspace_dtmp);
         }
         rr_matrix(RR_mat, V, ops, para, ws);
@@ -1249,6 +1524,11 @@ extern "C" void GCGE_WORKSPACE_Free(GCGE_WORKSPACE **workspace, GCGE_PARA *, GCG
     }
     for (int i = 0; i < 4; i++)
         if (ws->bcg32[i]) ops->MultiVecDestroy(&ws->bcg32[i], ws->bcg32_cols, ops);
+    if (ws->mixed32) {
+        GCGE_SOLVER *s32 = (GCGE_SOLVER *)ws->mixed32;
+        ws->mixed32 = nullptr;
+        GCGE_SOLVER_Free_All(&s32);
+    }
     free(ws);
     *workspace = nullptr;
 }
@@ -1378,11 +1658,29 @@ extern "C" void GCGE_SOLVER_SetOpsLinearSolverWorkspace(GCGE_SOLVER *s, void *w)
 extern "C" void GCGE_SOLVER_GetEigenvalues(GCGE_SOLVER *s, GCGE_DOUBLE **eval) { *eval = s->eval; }
 extern "C" void GCGE_SOLVER_GetEigenvectors(GCGE_SOLVER *s, void ***evec) { *evec = s->evec; }
 
+// The per-solve state starts as GCGE_PARA_Setup and GCGE_WORKSPACE_Setup
+// leave it, so a set-up solver can solve again (with new initial vectors or
+// new matrix values) without reallocating its workspace; for the first solve
+// this changes nothing.
 extern "C" void GCGE_SOLVER_Solve(GCGE_SOLVER *s)
 {
     if (!s->workspace || !s->workspace->V) {
         fprintf(stderr, "GCGE_SOLVER_Solve: solver is not set up (missing operations?)\n");
         return;
     }
+    GCGE_PARA *P = s->para;
+    GCGE_WORKSPACE *ws = s->workspace;
+    const int nev = P->nev;
+    P->num_iter = 0;
+    P->num_unlock = nev;
+    P->sum_res = 0.0;
+    for (int i = 0; i < nev; i++) P->res[i] = 0.0;
+    ws->dim_x = nev;
+    ws->dim_xp = 0;
+    ws->dim_xpw = nev;
+    ws->conv_bs = 0;
+    ws->num_soft_locked_in_last_iter = 0;
+    ws->unconv_bs = (nev < P->block_size) ? nev : P->block_size;
+    for (int i = 0; i < ws->max_dim_x; i++) ws->unlock[i] = i;
     GCGE_Solve(s->A, s->B, s->eval, s->evec, s->para, s->ops, s->workspace);
 }

@@ -18,6 +18,12 @@
 // locked pair can drift past the tolerance while the others converge; the
 // refinement therefore ends with a check of every pair and, while one is
 // above the tolerance, restarts from the current vectors (at most twice).
+// The FP32 phase (and the FP32 inner solves of the refinement) are skipped
+// when the Gershgorin bound of A (or B) lies outside [1e-18, 1e18], and the
+// phase's result is discarded when it ends with a
+// non-finite residual: the FP64 iteration then starts from the caller's
+// block.  The FP32 phase's solver and workspace stay with the caller's
+// workspace (mixed32) for later calls.
 #include <cfloat>
 #include <chrono>
 #include <cmath>
@@ -26,6 +32,7 @@
 #include <vector>
 
 #include "gcge_app_hip.h"
+#include "hip_common.h"
 
 namespace {
 double now_s()
@@ -33,9 +40,10 @@ double now_s()
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-// Largest residual of the solver's nev eigenpairs under its convergence
-// criterion (the formulas of GCGE_CheckConvergence).
-double max_residual(GCGE_SOLVER *solver)
+// Residuals of the solver's nev eigenpairs under its convergence criterion
+// (the formulas of GCGE_CheckConvergence) into res[]; returns the largest
+// (NaN when any is NaN).
+double pair_residuals(GCGE_SOLVER *solver, double *res)
 {
     GCGE_PARA *P = solver->para;
     const int nev = P->nev;
@@ -54,9 +62,18 @@ double max_residual(GCGE_SOLVER *solver)
         else if (std::strcmp(ct, "O") == 0) r = rn[i] / xn[i] / (P->conv_omega_norm + l);
         else if (ns) r = rn[i] / (l * bn[i]);
         else r = rn[i] / xn[i];
+        res[i] = r;
         if (!(r <= mx)) mx = r;  // NaN propagates
     }
     return mx;
+}
+
+// FP32 storage of A's values and of vectors scaled by them needs ||A|| (the
+// Gershgorin bound) well inside FP32's range: the Gram and CG products form
+// squares of A-scaled quantities.
+bool fp32_range_ok(const HIP_MAT *M)
+{
+    return !M || (M->row_abs_max >= 1e-18 && M->row_abs_max <= 1e18);
 }
 }  // namespace
 
@@ -91,96 +108,119 @@ extern "C" GCGE_INT GCGE_HIP_SOLVER_SolveMixed(GCGE_SOLVER *solver, const GCGE_H
     GCGE_HIP_MIXED_INFO inf;
     std::memset(&inf, 0, sizeof(inf));
 
-    // ---- FP32 phase
+    // ---- FP32 phase (skipped when FP32 cannot hold the scaled quantities)
     const double t0 = now_s();
-    HIP_MatSetPrecision(A, GCGE_HIP_FP32);
-    if (B) HIP_MatSetPrecision(B, GCGE_HIP_FP32);
-    GCGE_SOLVER *s32 = nullptr;
-    GCGE_SOLVER_Create(&s32);
-    {
-        // same parameters as the caller's solver, own tolerance and budget
-        GCGE_ORTH_PARA *op32 = s32->para->orth_para;
-        GCGE_STATISTIC_PARA *st32 = s32->para->stat_para;
-        double *res32 = s32->para->res;
-        std::memcpy(s32->para, P, sizeof(GCGE_PARA));
-        std::memcpy(op32, P->orth_para, sizeof(GCGE_ORTH_PARA));
-        s32->para->orth_para = op32;
-        s32->para->stat_para = st32;
-        s32->para->res = res32;
-        // GCGE_PARA_Setup adds DBL_EPSILON again: undo the caller's addition
-        op32->orth_zero_tol -= DBL_EPSILON;
-        op32->scbgs_reorth_tol -= DBL_EPSILON;
-        s32->para->ev_tol = o.fp32_tol;
-        s32->para->ev_max_it = o.fp32_max_it;
-        s32->para->stagnation_window = o.stagnation_window;
-        s32->para->stagnation_ratio = o.stagnation_ratio;
-        s32->para->stagnation_start = o.stagnation_start;
-        s32->para->print_para = 0;
-        s32->para->bcg_precision = 0;  // the phase is FP32 throughout
-        s32->para->basis_eps = FLT_EPSILON;
-        s32->para->cg_max_it = o.cg_max_it;
-        s32->para->bcg_precond = o.bcg_precond;
-        s32->para->rr_eig_full = o.rr_eig_full;
-        if (o.fp32_w_orth) s32->para->w_orth_type = (char *)"cholqr";
+    GCGE_WORKSPACE *ws = solver->workspace;
+    const bool run32 = fp32_range_ok(A) && fp32_range_ok(B);
+    bool use32 = false;  // the FP32 eigenvectors start the refinement
+    if (run32) {
+        GCGE_SOLVER *s32 = (GCGE_SOLVER *)ws->mixed32;
+        if (s32 && (s32->para->nev != nev || s32->para->block_size != P->block_size || s32->A != A ||
+                    s32->B != B)) {
+            ws->mixed32 = nullptr;
+            GCGE_SOLVER_Free_All(&s32);
+        }
+        const bool fresh = s32 == nullptr;
+        if (fresh) GCGE_SOLVER_Create(&s32);
+        {
+            // same parameters as the caller's solver, own tolerance and budget
+            GCGE_ORTH_PARA *op32 = s32->para->orth_para;
+            GCGE_STATISTIC_PARA *st32 = s32->para->stat_para;
+            double *res32 = s32->para->res;
+            std::memcpy(s32->para, P, sizeof(GCGE_PARA));
+            std::memcpy(op32, P->orth_para, sizeof(GCGE_ORTH_PARA));
+            s32->para->orth_para = op32;
+            s32->para->stat_para = st32;
+            s32->para->res = res32;
+            if (fresh) {
+                // GCGE_PARA_Setup adds DBL_EPSILON again: undo the caller's addition
+                op32->orth_zero_tol -= DBL_EPSILON;
+                op32->scbgs_reorth_tol -= DBL_EPSILON;
+            }
+            s32->para->ev_tol = o.fp32_tol;
+            s32->para->ev_max_it = o.fp32_max_it;
+            s32->para->stagnation_window = o.stagnation_window;
+            s32->para->stagnation_ratio = o.stagnation_ratio;
+            s32->para->stagnation_start = o.stagnation_start;
+            s32->para->print_para = 0;
+            s32->para->bcg_precision = 0;  // the phase is FP32 throughout
+            s32->para->basis_eps = FLT_EPSILON;
+            s32->para->cg_max_it = o.cg_max_it;
+            s32->para->bcg_precond = o.bcg_precond;
+            s32->para->rr_eig_full = o.rr_eig_full;
+            if (o.fp32_w_orth) s32->para->w_orth_type = (char *)"cholqr";
+        }
+        HIP_MatSetPrecision(A, GCGE_HIP_FP32);
+        if (B) HIP_MatSetPrecision(B, GCGE_HIP_FP32);
+        if (fresh) {
+            s32->eval = (double *)calloc(nev, sizeof(double));
+            void **evec32 = nullptr;
+            HIP_MultiVecCreate(&evec32, nev, A);
+            GCGE_SOLVER_SetMatA(s32, A);
+            if (B) GCGE_SOLVER_SetMatB(s32, B);
+            GCGE_SOLVER_SetHIPOps(s32);
+            GCGE_SOLVER_SetEigenvalues(s32, s32->eval);
+            GCGE_SOLVER_SetEigenvectors(s32, evec32);
+            GCGE_SOLVER_Setup(s32);
+            ws->mixed32 = s32;
+        }
+        if (P->given_init_evec) {
+            GCGE_INT s[2] = {0, 0}, e[2] = {nev, nev};
+            solver->ops->MultiVecCopy(solver->evec, s32->evec, s, e, solver->ops);
+        }
+        GCGE_SOLVER_Solve(s32);
+        GCGE_HIP_Synchronize();
+        inf.fp32_iters = s32->para->num_iter;
+        HIP_MatSetPrecision(A, GCGE_HIP_FP64);
+        if (B) HIP_MatSetPrecision(B, GCGE_HIP_FP64);
+        const int err = GCGE_HIP_GetLastError(nullptr, 0);
+        if (err == 190 || (err == 0 && !std::isfinite(s32->para->max_res))) {
+            // the FP32 iterates went non-finite: start the FP64 solve from the caller's block
+            GCGE_HIP_ClearError();
+            inf.fp32_failed = 1;
+        } else if (err == 0) {
+            GCGE_INT s[2] = {0, 0}, e[2] = {nev, nev};
+            solver->ops->MultiVecCopy(s32->evec, solver->evec, s, e, solver->ops);
+            use32 = true;
+        }
+    } else {
+        inf.fp32_failed = 2;
     }
-    s32->eval = (double *)calloc(nev, sizeof(double));
-    void **evec32 = nullptr;
-    HIP_MultiVecCreate(&evec32, nev, A);
-    if (P->given_init_evec) {
-        GCGE_INT s[2] = {0, 0}, e[2] = {nev, nev};
-        solver->ops->MultiVecCopy(solver->evec, evec32, s, e, solver->ops);
-    }
-    GCGE_SOLVER_SetMatA(s32, A);
-    if (B) GCGE_SOLVER_SetMatB(s32, B);
-    GCGE_SOLVER_SetHIPOps(s32);
-    GCGE_SOLVER_SetEigenvalues(s32, s32->eval);
-    GCGE_SOLVER_SetEigenvectors(s32, evec32);
-    GCGE_SOLVER_Setup(s32);
-    GCGE_SOLVER_Solve(s32);
-    GCGE_HIP_Synchronize();
-    inf.fp32_iters = s32->para->num_iter;
     inf.fp32_seconds = now_s() - t0;
 
-    // ---- FP64 refinement from the FP32 eigenvectors
-    HIP_MatSetPrecision(A, GCGE_HIP_FP64);
-    if (B) HIP_MatSetPrecision(B, GCGE_HIP_FP64);
-    {
-        GCGE_INT s[2] = {0, 0}, e[2] = {nev, nev};
-        solver->ops->MultiVecCopy(evec32, solver->evec, s, e, solver->ops);
-    }
-    GCGE_SOLVER_Free_All(&s32);
+    // ---- FP64 refinement from the FP32 eigenvectors (or the caller's start)
     const double t1 = now_s();
     const int given = P->given_init_evec, maxit = P->ev_max_it, bcgp = P->bcg_precision, cgit = P->cg_max_it,
               pcnd = P->bcg_precond, rrf = P->rr_eig_full;
     char *worth = P->w_orth_type;
-    P->given_init_evec = 1;
+    P->given_init_evec = use32 ? 1 : given;
     P->ev_max_it = o.fp64_max_it;
-    P->bcg_precision = o.refine_bcg_precision;
+    P->bcg_precision = run32 ? o.refine_bcg_precision : 0;  // FP32 inner solves need the same range
     P->cg_max_it = o.cg_max_it;
     P->bcg_precond = o.bcg_precond;
     P->rr_eig_full = o.rr_eig_full;
     if (o.refine_w_orth) P->w_orth_type = (char *)"cholqr";
-    int iters = 0, bcg_iters = 0, bcg_calls = 0;
+    int iters = 0, bcg_iters = 0, bcg_calls = 0, failing = nev;
+    bool nonfinite = false;
+    std::vector<double> res(nev);
     for (int pass = 0; pass < 3; pass++) {
-        P->num_iter = 0;
-        P->num_unlock = nev;
-        P->sum_res = 0.0;
-        for (int i = 0; i < nev; i++) P->res[i] = 1.0;
-        // the workspace bookkeeping restarts as after GCGE_WORKSPACE_Setup
-        GCGE_WORKSPACE *ws = solver->workspace;
-        ws->dim_x = nev;
-        ws->dim_xp = 0;
-        ws->dim_xpw = nev;
-        ws->num_soft_locked_in_last_iter = 0;
-        ws->unconv_bs = (nev < P->block_size) ? nev : P->block_size;
-        for (int i = 0; i < ws->max_dim_x; i++) ws->unlock[i] = i;
         GCGE_SOLVER_Solve(solver);
         GCGE_HIP_Synchronize();
+        inf.refine_passes = pass + 1;
         iters += P->num_iter;
         bcg_iters += P->bcg_iters_total;
         bcg_calls += P->bcg_calls;
-        if (P->num_unlock != 0 || GCGE_HIP_GetLastError(nullptr, 0) != 0) break;  // budget spent, or an error
-        if (max_residual(solver) <= P->ev_tol) break;
+        if (GCGE_HIP_GetLastError(nullptr, 0) != 0) break;
+        // every pair under the caller's criterion (locked pairs may have drifted)
+        pair_residuals(solver, res.data());
+        failing = 0;
+        nonfinite = false;
+        for (int i = 0; i < nev; i++) {
+            if (!(res[i] <= P->ev_tol)) failing++;
+            if (!std::isfinite(res[i])) nonfinite = true;
+        }
+        if (failing == 0 || nonfinite || P->num_unlock != 0) break;  // done, broken, or budget spent
+        P->given_init_evec = 1;  // restart from the current vectors
     }
     P->num_iter = iters;  // the statistics of the passes added up
     P->bcg_iters_total = bcg_iters;
@@ -194,7 +234,26 @@ extern "C" GCGE_INT GCGE_HIP_SOLVER_SolveMixed(GCGE_SOLVER *solver, const GCGE_H
     P->w_orth_type = worth;
     inf.fp64_iters = P->num_iter;
     inf.fp64_seconds = now_s() - t1;
+    if (GCGE_HIP_GetLastError(nullptr, 0) == 0) {
+        // the result is what the final check saw
+        double mx = 0.0;
+        for (int i = 0; i < nev; i++) {
+            P->res[i] = res[i];
+            if (!(res[i] <= mx)) mx = res[i];
+        }
+        P->max_res = mx;
+        P->num_unlock = failing;
+        if (nonfinite) {
+            for (int i = 0; i < nev; i++)
+                if (!std::isfinite(res[i])) {
+                    gh::latch(190, "GCGE_HIP_SOLVER_SolveMixed: residual of eigenpair %d is %g", i, res[i]);
+                    break;
+                }
+        }
+    }
     inf.fp64_num_unlock = P->num_unlock;
     if (info) *info = inf;
-    return GCGE_HIP_GetLastError(msg, (GCGE_INT)sizeof(msg));
+    const int err = GCGE_HIP_GetLastError(msg, (GCGE_INT)sizeof(msg));
+    if (err) return err;
+    return P->num_unlock ? GCGE_HIP_NOT_CONVERGED : 0;
 }

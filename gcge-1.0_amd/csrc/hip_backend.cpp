@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -392,6 +393,7 @@ using namespace gh;
 
 // ====================================================================== matrices
 static int pick_gather_rows(int n, const int *rowptr, const int *col);
+static bool build_halo(HIP_MAT *A, const int *rowptr, const int *col);
 
 extern "C" HIP_MAT *HIP_MatCreateFromCSR(GCGE_INT nrows, GCGE_INT ncols, GCGE_INT nnz, const GCGE_INT *rowptr,
                                          const GCGE_INT *col, const GCGE_DOUBLE *val)
@@ -449,12 +451,68 @@ extern "C" HIP_MAT *HIP_MatCreateFromCSR(GCGE_INT nrows, GCGE_INT ncols, GCGE_IN
         ok = check(hipMalloc((void **)&A->long_rows, sizeof(int) * A->nlong), "hipMalloc long_rows") &&
              check(hipMemcpy(A->long_rows, longs.data(), sizeof(int) * A->nlong, hipMemcpyHostToDevice),
                    "H2D long_rows");
+    if (ok && !A->gather_rows) ok = build_halo(A, rowptr, col);
     if (!ok) {
         HIP_MatDestroy(&A);
         return nullptr;
     }
     g_ctx.dev_bytes += sizeof(int) * (double)(nrows + 1) + 12.0 * nnz;
     return A;
+}
+
+// Staged row blocks of k_spmm_halo: for each 256-row block, the rows its
+// entries read besides its own (ascending), and every entry's LDS slot.
+// Built with GCGE_HIP_SPMM_GATHER=halo when no block reads more than
+// kHaloCap - 256 other rows; rows longer than kLongRow (k_spmm_long) are
+// left out.
+static bool build_halo(HIP_MAT *A, const int *rowptr, const int *col)
+{
+    const char *e = getenv("GCGE_HIP_SPMM_GATHER");
+    if (!e || strcmp(e, "halo")) return true;  // measured slower than k_spmm on the 3-D stencil: opt-in
+    const int n = A->n, R = 256, nblk = (n + R - 1) / R;
+    std::vector<int> hptr((size_t)nblk + 1, 0), hcol, ext;
+    std::vector<unsigned short> lcol((size_t)std::max(1, A->nnz), 0);
+    int most = 0;
+    for (int b = 0; b < nblk; b++) {
+        const int r0 = b * R, r1 = std::min(n, r0 + R);
+        ext.clear();
+        for (int r = r0; r < r1; r++) {
+            if (rowptr[r + 1] - rowptr[r] > kLongRow) continue;
+            for (int q = rowptr[r]; q < rowptr[r + 1]; q++)
+                if (col[q] < r0 || col[q] >= r1) ext.push_back(col[q]);
+        }
+        std::sort(ext.begin(), ext.end());
+        ext.erase(std::unique(ext.begin(), ext.end()), ext.end());
+        if ((int)ext.size() > kHaloCap - R) return true;  // too wide for LDS: keep k_spmm
+        most = std::max(most, (int)ext.size());
+        for (int r = r0; r < r1; r++) {
+            if (rowptr[r + 1] - rowptr[r] > kLongRow) continue;
+            for (int q = rowptr[r]; q < rowptr[r + 1]; q++) {
+                const int c = col[q];
+                lcol[q] = (unsigned short)((c >= r0 && c < r1)
+                                               ? c - r0
+                                               : R + (int)(std::lower_bound(ext.begin(), ext.end(), c) - ext.begin()));
+            }
+        }
+        hcol.insert(hcol.end(), ext.begin(), ext.end());
+        hptr[b + 1] = (int)hcol.size();
+    }
+    const size_t nh = std::max<size_t>(1, hcol.size());
+    bool ok = check(hipMalloc((void **)&A->halo_ptr, sizeof(int) * hptr.size()), "hipMalloc halo_ptr") &&
+              check(hipMalloc((void **)&A->halo_col, sizeof(int) * nh), "hipMalloc halo_col") &&
+              check(hipMalloc((void **)&A->lcol, sizeof(unsigned short) * lcol.size()), "hipMalloc lcol");
+    if (!ok) return false;
+    ok = check(hipMemcpy(A->halo_ptr, hptr.data(), sizeof(int) * hptr.size(), hipMemcpyHostToDevice), "H2D halo") &&
+         (hcol.empty() ||
+          check(hipMemcpy(A->halo_col, hcol.data(), sizeof(int) * hcol.size(), hipMemcpyHostToDevice), "H2D halo")) &&
+         check(hipMemcpy(A->lcol, lcol.data(), sizeof(unsigned short) * lcol.size(), hipMemcpyHostToDevice),
+               "H2D lcol");
+    if (!ok) return false;
+    A->halo = 1;
+    A->halo_cap = ((R + most + 31) / 32) * 32;
+    A->halo_bytes = 4.0 * (double)(hptr.size() + nh) + 2.0 * (double)lcol.size();
+    g_ctx.dev_bytes += A->halo_bytes;
+    return true;
 }
 
 // Gather layout of the SpMM.  The row-major path is used when
@@ -467,13 +525,14 @@ extern "C" HIP_MAT *HIP_MatCreateFromCSR(GCGE_INT nrows, GCGE_INT ncols, GCGE_IN
 //   indices fall in; a stencil's rows share a few lines (about 0.03 distinct
 //   32-column segments per entry over a 64-row block), a random graph's do
 //   not (0.4 for a Chung-Lu graph); the threshold is 0.15.
-// GCGE_HIP_SPMM_GATHER=rows|columns overrides the choice for matrices
-// created afterwards.
+// GCGE_HIP_SPMM_GATHER=rows|columns|halo overrides the choice for matrices
+// created afterwards (columns: the direct-gather k_spmm; halo: the staged
+// row blocks of k_spmm_halo where they fit).
 static int pick_gather_rows(int n, const int *rowptr, const int *col)
 {
     const char *e = getenv("GCGE_HIP_SPMM_GATHER");
     if (e && !strcmp(e, "rows")) return 1;
-    if (e && !strcmp(e, "columns")) return 0;
+    if (e && (!strcmp(e, "columns") || !strcmp(e, "halo"))) return 0;
     if ((double)rowptr[n] > 8.0 * n) return 1;
     const int nb = n / 64;
     if (nb == 0) return 0;
@@ -490,31 +549,162 @@ static int pick_gather_rows(int n, const int *rowptr, const int *col)
     return ents > 0.0 && lines > 0.15 * ents ? 1 : 0;
 }
 
+// ---- matrix files: the text format of CSR_ReadMatFile (app/csr/gcge_app_csr.c:185-236)
+// and a binary form of the same arrays ("GCGECSR1", int32 nrow, ncol, nnz,
+// int32 0, rowptr[nrow+1] int32, col[nnz] int32, val[nnz] f64, little endian)
+static const char kCsrMagic[8] = {'G', 'C', 'G', 'E', 'C', 'S', 'R', '1'};
+
+namespace {
+struct CsrHost {
+    int nrow = 0, ncol = 0, nnz = 0;
+    std::vector<int> ia, ja;
+    std::vector<double> aa;
+};
+
+bool read_all(FILE *f, std::vector<char> &buf)
+{
+    if (fseek(f, 0, SEEK_END) != 0) return false;
+    const long sz = ftell(f);
+    if (sz < 0 || fseek(f, 0, SEEK_SET) != 0) return false;
+    buf.resize((size_t)sz + 1);
+    if (sz > 0 && fread(buf.data(), 1, (size_t)sz, f) != (size_t)sz) return false;
+    buf[(size_t)sz] = '\0';
+    return true;
+}
+
+// whitespace-separated numbers, parsed in place (strtol / strtod: the values
+// fscanf's %d / %lf give)
+bool parse_text(const std::vector<char> &buf, CsrHost &m, const char *filename)
+{
+    const char *p = buf.data();
+    char *q = nullptr;
+    long v[3];
+    for (int i = 0; i < 3; i++) {
+        v[i] = strtol(p, &q, 10);
+        if (q == p) {
+            latch(121, "HIP_ReadMatFile: bad header in %s", filename);
+            return false;
+        }
+        p = q;
+    }
+    if (v[0] <= 0 || v[1] <= 0 || v[2] < 0 || v[0] > INT32_MAX - 1 || v[2] > INT32_MAX) {
+        latch(121, "HIP_ReadMatFile: bad header in %s", filename);
+        return false;
+    }
+    m.nrow = (int)v[0];
+    m.ncol = (int)v[1];
+    m.nnz = (int)v[2];
+    m.ia.resize((size_t)m.nrow + 1);
+    m.ja.resize((size_t)m.nnz);
+    m.aa.resize((size_t)m.nnz);
+    for (size_t i = 0; i < m.ia.size(); i++, p = q) {
+        m.ia[i] = (int)strtol(p, &q, 10);
+        if (q == p) goto truncated;
+    }
+    for (size_t i = 0; i < m.ja.size(); i++, p = q) {
+        m.ja[i] = (int)strtol(p, &q, 10);
+        if (q == p) goto truncated;
+    }
+    for (size_t i = 0; i < m.aa.size(); i++, p = q) {
+        m.aa[i] = strtod(p, &q);
+        if (q == p) goto truncated;
+    }
+    return true;
+truncated:
+    latch(122, "HIP_ReadMatFile: truncated file %s", filename);
+    return false;
+}
+
+bool parse_binary(const std::vector<char> &buf, CsrHost &m, const char *filename)
+{
+    const size_t hdr = 8 + 4 * sizeof(int);
+    if (buf.size() - 1 < hdr) {
+        latch(122, "HIP_ReadMatFile: truncated file %s", filename);
+        return false;
+    }
+    int h[4];
+    std::memcpy(h, buf.data() + 8, sizeof(h));
+    if (h[0] <= 0 || h[1] <= 0 || h[2] < 0) {
+        latch(121, "HIP_ReadMatFile: bad header in %s", filename);
+        return false;
+    }
+    m.nrow = h[0];
+    m.ncol = h[1];
+    m.nnz = h[2];
+    const size_t need = hdr + sizeof(int) * ((size_t)m.nrow + 1) + sizeof(int) * (size_t)m.nnz +
+                        sizeof(double) * (size_t)m.nnz;
+    if (buf.size() - 1 < need) {
+        latch(122, "HIP_ReadMatFile: truncated file %s", filename);
+        return false;
+    }
+    const char *p = buf.data() + hdr;
+    m.ia.resize((size_t)m.nrow + 1);
+    m.ja.resize((size_t)m.nnz);
+    m.aa.resize((size_t)m.nnz);
+    std::memcpy(m.ia.data(), p, sizeof(int) * m.ia.size());
+    p += sizeof(int) * m.ia.size();
+    std::memcpy(m.ja.data(), p, sizeof(int) * m.ja.size());
+    p += sizeof(int) * m.ja.size();
+    std::memcpy(m.aa.data(), p, sizeof(double) * m.aa.size());
+    return true;
+}
+}  // namespace
+
 extern "C" HIP_MAT *HIP_ReadMatFile(const char *filename)
 {
-    FILE *f = fopen(filename, "r");
+    FILE *f = filename ? fopen(filename, "rb") : nullptr;
     if (!f) {
-        latch(120, "HIP_ReadMatFile: cannot open %s", filename);
+        latch(120, "HIP_ReadMatFile: cannot open %s", filename ? filename : "(null)");
         return nullptr;
     }
-    int nrow = 0, ncol = 0, nnz = 0;
-    if (fscanf(f, "%d %d %d", &nrow, &ncol, &nnz) != 3 || nrow <= 0 || nnz < 0) {
-        fclose(f);
-        latch(121, "HIP_ReadMatFile: bad header in %s", filename);
-        return nullptr;
-    }
-    std::vector<int> ia(nrow + 1), ja(nnz);
-    std::vector<double> aa(nnz);
-    bool ok = true;
-    for (int i = 0; i <= nrow && ok; i++) ok = fscanf(f, "%d", &ia[i]) == 1;
-    for (int i = 0; i < nnz && ok; i++) ok = fscanf(f, "%d", &ja[i]) == 1;
-    for (int i = 0; i < nnz && ok; i++) ok = fscanf(f, "%lf", &aa[i]) == 1;
+    std::vector<char> buf;
+    const bool ok = read_all(f, buf);
     fclose(f);
     if (!ok) {
-        latch(122, "HIP_ReadMatFile: truncated file %s", filename);
+        latch(120, "HIP_ReadMatFile: cannot read %s", filename);
         return nullptr;
     }
-    return HIP_MatCreateFromCSR(nrow, ncol, nnz, ia.data(), ja.data(), aa.data());
+    CsrHost m;
+    const bool binary = buf.size() - 1 >= 8 && std::memcmp(buf.data(), kCsrMagic, 8) == 0;
+    if (!(binary ? parse_binary(buf, m, filename) : parse_text(buf, m, filename))) return nullptr;
+    buf.clear();
+    buf.shrink_to_fit();
+    return HIP_MatCreateFromCSR(m.nrow, m.ncol, m.nnz, m.ia.data(), m.ja.data(), m.aa.data());
+}
+
+extern "C" GCGE_INT HIP_WriteMatFile(const char *filename, GCGE_INT nrows, GCGE_INT ncols, GCGE_INT nnz,
+                                     const GCGE_INT *rowptr, const GCGE_INT *col, const GCGE_DOUBLE *val,
+                                     GCGE_INT binary)
+{
+    if (!filename || nrows <= 0 || ncols <= 0 || nnz < 0 || !rowptr || (nnz > 0 && (!col || !val))) {
+        latch(123, "HIP_WriteMatFile: bad arguments");
+        return 123;
+    }
+    FILE *f = fopen(filename, binary ? "wb" : "w");
+    if (!f) {
+        latch(124, "HIP_WriteMatFile: cannot create %s", filename);
+        return 124;
+    }
+    bool ok = true;
+    if (binary) {
+        const int h[4] = {nrows, ncols, nnz, 0};
+        ok = fwrite(kCsrMagic, 1, 8, f) == 8 && fwrite(h, sizeof(int), 4, f) == 4 &&
+             fwrite(rowptr, sizeof(int), (size_t)nrows + 1, f) == (size_t)nrows + 1 &&
+             fwrite(col, sizeof(int), (size_t)nnz, f) == (size_t)nnz &&
+             fwrite(val, sizeof(double), (size_t)nnz, f) == (size_t)nnz;
+    } else {
+        // one number per line, values with 17 significant digits (they read back exactly)
+        ok = fprintf(f, "%d\n%d\n%d\n", nrows, ncols, nnz) > 0;
+        for (int i = 0; ok && i <= nrows; i++) ok = fprintf(f, "%d\n", rowptr[i]) > 0;
+        for (int i = 0; ok && i < nnz; i++) ok = fprintf(f, "%d\n", col[i]) > 0;
+        for (int i = 0; ok && i < nnz; i++) ok = fprintf(f, "%.17g\n", val[i]) > 0;
+    }
+    ok = (fclose(f) == 0) && ok;
+    if (!ok) {
+        latch(124, "HIP_WriteMatFile: write to %s failed", filename);
+        return 124;
+    }
+    return 0;
 }
 
 extern "C" void HIP_MatDestroy(HIP_MAT **mat)
@@ -531,8 +721,11 @@ extern "C" void HIP_MatDestroy(HIP_MAT **mat)
         if (A->dinv64) (void)hipFree(A->dinv64);
         if (A->long_rows) (void)hipFree(A->long_rows);
         if (A->dinv32) (void)hipFree(A->dinv32);
+        if (A->halo_ptr) (void)hipFree(A->halo_ptr);
+        if (A->halo_col) (void)hipFree(A->halo_col);
+        if (A->lcol) (void)hipFree(A->lcol);
         g_ctx.dev_bytes -= sizeof(int) * (double)(A->n + 1) + 12.0 * A->nnz + (A->val32 ? 4.0 * A->nnz : 0.0) +
-                           (A->rowidx ? 4.0 * A->nnz : 0.0);
+                           (A->rowidx ? 4.0 * A->nnz : 0.0) + A->halo_bytes;
         A->magic = 0;
     }
     delete A;
@@ -548,8 +741,22 @@ extern "C" GCGE_INT HIP_MatSetValues(HIP_MAT *A, const GCGE_DOUBLE *val)
         latch(114, "HIP_MatSetValues: bad arguments");
         return 114;
     }
+    if (!sync()) return g_err;  // kernels queued on the backend stream may still read the old values
     if (!check(hipMemcpy(A->val64, val, sizeof(double) * A->nnz, hipMemcpyHostToDevice), "H2D val")) return g_err;
+    // everything derived from the values follows them: the FP32 copy, the
+    // Jacobi diagonal of the preconditioned inner solves, the Gershgorin bound
     if (A->val32) launch_to_f32(A->nnz, A->val64, A->val32);
+    if (A->dinv64) launch_diag_inv(A);
+    std::vector<int> rp((size_t)A->n + 1);
+    if (!check(hipMemcpy(rp.data(), A->rowptr, sizeof(int) * rp.size(), hipMemcpyDeviceToHost), "D2H rowptr"))
+        return g_err;
+    double rowabs = 0.0;
+    for (int i = 0; i < A->n; i++) {
+        double sum = 0.0;
+        for (int e = rp[i]; e < rp[i + 1]; e++) sum += std::fabs(val[e]);
+        rowabs = std::max(rowabs, sum);
+    }
+    A->row_abs_max = rowabs;
     return g_err;
 }
 
@@ -1320,49 +1527,95 @@ extern "C" GCGE_INT GCGE_HIP_AbiLayout(GCGE_INT *out, GCGE_INT n)
 }
 
 // ---- eigenvalue gradient (adjoint) ----
-extern "C" GCGE_INT GCGE_HIP_EigGradient(HIP_MAT *A, void **X, GCGE_INT k, const GCGE_DOUBLE *w, GCGE_INT nparam,
-                                         const GCGE_INT *jrowptr, const GCGE_INT *jnz, const GCGE_DOUBLE *jval,
-                                         GCGE_DOUBLE *grad)
+namespace {
+
+// A Jacobian of a matrix's stored values: nparam rows, jrowptr ascending,
+// every jnz a stored-entry index of the matrix.
+bool jacobian_ok(const HIP_MAT *M, GCGE_INT nparam, const GCGE_INT *jrowptr, const GCGE_INT *jnz,
+                 const GCGE_DOUBLE *jval, const char *which)
 {
-    MV *mx = as_mv(X);
-    if (!A || A->magic != kMagicMat || !range_ok(mx, 0, k, "GCGE_HIP_EigGradient") ||
-        !mat_ok(A, mx, "GCGE_HIP_EigGradient") || nparam < 0 || !jrowptr || !grad) {
-        latch(180, "GCGE_HIP_EigGradient: bad arguments");
-        return g_err;
+    if (!jrowptr || (jrowptr[nparam] > 0 && (!jnz || !jval)) || jrowptr[0] != 0) {
+        latch(180, "GCGE_HIP_EigGradient: bad %s Jacobian arrays", which);
+        return false;
     }
-    const int nj = jrowptr[nparam];
     for (int p = 0; p < nparam; p++)
         if (jrowptr[p + 1] < jrowptr[p]) {
-            latch(181, "GCGE_HIP_EigGradient: jrowptr decreases at %d", p);
-            return g_err;
+            latch(181, "GCGE_HIP_EigGradient: %s jrowptr decreases at %d", which, p);
+            return false;
         }
-    for (int t = 0; t < nj; t++)
-        if (jnz[t] < 0 || jnz[t] >= A->nnz) {
-            latch(182, "GCGE_HIP_EigGradient: jnz[%d]=%d outside [0,%d)", t, jnz[t], A->nnz);
-            return g_err;
+    for (int t = 0; t < jrowptr[nparam]; t++)
+        if (jnz[t] < 0 || jnz[t] >= M->nnz) {
+            latch(182, "GCGE_HIP_EigGradient: %s jnz[%d]=%d outside [0,%d)", which, t, jnz[t], M->nnz);
+            return false;
         }
-    // device buffers: g[nnz], jrowptr, jnz, jval, grad
-    double *g = nullptr, *dval = nullptr, *dgrad = nullptr;
+    return true;
+}
+
+// grad (device, nparam) = scale * J^T g_M + (accumulate ? grad : 0), where g_M[e] =
+// sum_i wgt_i x_i[row_e] x_i[col_e] over the stored entries of M
+bool gradient_term(HIP_MAT *M, MV *mx, GCGE_INT k, const double *wgt, GCGE_INT nparam, const GCGE_INT *jrowptr,
+                   const GCGE_INT *jnz, const GCGE_DOUBLE *jval, double scale, bool accumulate, double *dgrad)
+{
+    const int nj = jrowptr[nparam];
+    double *g = nullptr, *dval = nullptr;
     int *drp = nullptr, *dnz = nullptr;
-    bool ok = check(hipMalloc((void **)&g, sizeof(double) * std::max(1, A->nnz)), "hipMalloc g") &&
+    bool ok = check(hipMalloc((void **)&g, sizeof(double) * std::max(1, M->nnz)), "hipMalloc g") &&
               check(hipMalloc((void **)&drp, sizeof(int) * (nparam + 1)), "hipMalloc jrowptr") &&
               check(hipMalloc((void **)&dnz, sizeof(int) * std::max(1, nj)), "hipMalloc jnz") &&
-              check(hipMalloc((void **)&dval, sizeof(double) * std::max(1, nj)), "hipMalloc jval") &&
-              check(hipMalloc((void **)&dgrad, sizeof(double) * std::max(1, nparam)), "hipMalloc grad");
+              check(hipMalloc((void **)&dval, sizeof(double) * std::max(1, nj)), "hipMalloc jval");
     if (ok)
         ok = check(hipMemcpy(drp, jrowptr, sizeof(int) * (nparam + 1), hipMemcpyHostToDevice), "H2D") &&
              (nj == 0 || check(hipMemcpy(dnz, jnz, sizeof(int) * nj, hipMemcpyHostToDevice), "H2D")) &&
              (nj == 0 || check(hipMemcpy(dval, jval, sizeof(double) * nj, hipMemcpyHostToDevice), "H2D"));
-    if (ok) ok = sampled_outer(mx->prec, A, mx->data, mx->ld, k, w, g);
+    if (ok) ok = sampled_outer(mx->prec, M, mx->data, mx->ld, k, wgt, g);
     if (ok) {
-        launch_csr_gather_sum(nparam, drp, dnz, dval, g, dgrad);
-        ok = sync() && check(hipMemcpy(grad, dgrad, sizeof(double) * nparam, hipMemcpyDeviceToHost), "D2H grad");
+        launch_csr_gather_sum(nparam, drp, dnz, dval, g, dgrad, scale, accumulate);
+        ok = sync();
     }
     sync();
     if (g) (void)hipFree(g);
     if (drp) (void)hipFree(drp);
     if (dnz) (void)hipFree(dnz);
     if (dval) (void)hipFree(dval);
-    if (dgrad) (void)hipFree(dgrad);
+    return ok;
+}
+
+}  // namespace
+
+extern "C" GCGE_INT GCGE_HIP_EigGradientGeneralized(HIP_MAT *A, HIP_MAT *B, void **X, GCGE_INT k,
+                                                    const GCGE_DOUBLE *lambda, const GCGE_DOUBLE *w, GCGE_INT nparam,
+                                                    const GCGE_INT *ja_rowptr, const GCGE_INT *ja_nz,
+                                                    const GCGE_DOUBLE *ja_val, const GCGE_INT *jb_rowptr,
+                                                    const GCGE_INT *jb_nz, const GCGE_DOUBLE *jb_val, GCGE_DOUBLE *grad)
+{
+    MV *mx = as_mv(X);
+    if (!A || A->magic != kMagicMat || !range_ok(mx, 0, k, "GCGE_HIP_EigGradient") ||
+        !mat_ok(A, mx, "GCGE_HIP_EigGradient") || nparam < 0 || !grad || !w ||
+        (B && (B->magic != kMagicMat || B->n != A->n)) || (B && jb_rowptr && !lambda)) {
+        latch(180, "GCGE_HIP_EigGradient: bad arguments");
+        return g_err;
+    }
+    if (!jacobian_ok(A, nparam, ja_rowptr, ja_nz, ja_val, "A")) return g_err;
+    const bool bterm = B && jb_rowptr;
+    if (bterm && !jacobian_ok(B, nparam, jb_rowptr, jb_nz, jb_val, "B")) return g_err;
+    double *dgrad = nullptr;
+    if (!check(hipMalloc((void **)&dgrad, sizeof(double) * std::max(1, nparam)), "hipMalloc grad")) return g_err;
+    // x_i^T (dA/dtheta) x_i weighted by w_i, then - lambda_i x_i^T (dB/dtheta) x_i
+    bool ok = gradient_term(A, mx, k, w, nparam, ja_rowptr, ja_nz, ja_val, 1.0, false, dgrad);
+    if (ok && bterm) {
+        std::vector<double> wl(k);
+        for (int i = 0; i < k; i++) wl[i] = w[i] * lambda[i];
+        ok = gradient_term(B, mx, k, wl.data(), nparam, jb_rowptr, jb_nz, jb_val, -1.0, true, dgrad);
+    }
+    if (ok) check(hipMemcpy(grad, dgrad, sizeof(double) * nparam, hipMemcpyDeviceToHost), "D2H grad");
+    (void)hipFree(dgrad);
     return g_err;
+}
+
+extern "C" GCGE_INT GCGE_HIP_EigGradient(HIP_MAT *A, void **X, GCGE_INT k, const GCGE_DOUBLE *w, GCGE_INT nparam,
+                                         const GCGE_INT *jrowptr, const GCGE_INT *jnz, const GCGE_DOUBLE *jval,
+                                         GCGE_DOUBLE *grad)
+{
+    return GCGE_HIP_EigGradientGeneralized(A, nullptr, X, k, nullptr, w, nparam, jrowptr, jnz, jval, nullptr, nullptr,
+                                           nullptr, grad);
 }

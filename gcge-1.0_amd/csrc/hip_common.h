@@ -68,7 +68,18 @@ struct HIP_MAT_ {
     int max_row_nnz;
     int gather_rows;  // SpMM gathers from a row-major copy of X (scattered column indices)
     double row_abs_max;  // max_i sum_j |a_ij| (Gershgorin bound of ||A||_2)
+    // staged-row-block SpMM (k_spmm_halo), chosen at creation when every
+    // 256-row block reads at most kHaloCap - 256 rows besides its own
+    int halo;                 // 1: SpMM by k_spmm_halo
+    int halo_cap;             // LDS slots per column: 256 own rows + the most extras of any block (rounded)
+    int *halo_ptr;            // device, nblk+1: each block's extras in halo_col
+    int *halo_col;            // device: the other rows a block's entries read, ascending
+    unsigned short *lcol;     // device, nnz: each entry's slot (own row r0+i -> i, extra j -> 256+j)
+    double halo_bytes;        // device bytes of the three arrays
 };
+
+// largest LDS slot count per column of a k_spmm_halo row block
+constexpr int kHaloCap = 4096;
 
 namespace gh {
 
@@ -194,8 +205,9 @@ bool lincomb(int prec, int64_t n, const void *X, int64_t ldx, int p, void *Y, in
 // per-nonzero sampled outer product g_e = sum_i w_i x_i[row_e] x_i[col_e]
 bool sampled_outer(int prec, HIP_MAT *A, const void *X, int64_t ldx, int k, const double *w,
                    double *g_dev);
-// grad[p] = sum_t jval[t] g[jnz[t]] over t in [jrowptr[p], jrowptr[p+1])
+// grad[p] = scale * sum_t jval[t] g[jnz[t]] over t in [jrowptr[p], jrowptr[p+1]),
+// added to grad[p] when accumulate
 void launch_csr_gather_sum(int nparam, const int *jrowptr, const int *jnz, const double *jval,
-                           const double *g, double *grad);
+                           const double *g, double *grad, double scale, bool accumulate);
 
 }  // namespace gh

@@ -369,6 +369,96 @@ __global__ __launch_bounds__(kSpmmRows) void k_spmm_long(const int *__restrict__
     }
 }
 
+// ------------------------------------------------- SpMM, staged row blocks
+// The column-gather kernel above reads, for every stored entry and column,
+// X(col, j) straight from memory; on a stencil the seven gathers of a row hit
+// the same few cache lines, but the L1 keeps almost none of them, so each
+// column costs about eight times its own bytes in L2 requests.  Here a row
+// block's workgroup first copies every X row its entries read -- its own 256
+// rows, then the other rows its entries reference ("extras", listed once per
+// block at matrix creation, sorted) -- for KC columns into LDS, with
+// coalesced loads, and the rows then read their entries' values from LDS
+// through 16-bit local column indices (also built at creation).  Each row's
+// first kHaloMaxE entries (local indices and values) stay in registers for all
+// columns.  Every sum still runs over the row in CSR order from 0, so Y is
+// bit-identical to k_spmm's.  Rows longer than long_thr are left to
+// k_spmm_long.  Row blocks are numbered so that each XCD walks a contiguous
+// range of rows (a stencil's plane neighbours stay in its L2).
+constexpr int kHaloMaxE = 8;
+
+template <typename T, bool DOT, int KC>
+__global__ __launch_bounds__(kSpmmRows) void k_spmm_halo(int n, const int *__restrict__ rowptr,
+                                                         const unsigned short *__restrict__ lcol,
+                                                         const T *__restrict__ val, const int *__restrict__ hptr,
+                                                         const int *__restrict__ hcol, const T *__restrict__ X,
+                                                         int64_t ldx, T *__restrict__ Y, int64_t ldy, int k,
+                                                         ColScalars g, double *__restrict__ partial, int nblk,
+                                                         int long_thr, int pstride, int hcap)
+{
+    extern __shared__ __align__(16) unsigned char halo_smem[];
+    T *s_x = reinterpret_cast<T *>(halo_smem);                    // [KC][hcap]
+    int *s_h = reinterpret_cast<int *>(s_x + (size_t)KC * hcap);  // [hcap - kSpmmRows]
+    __shared__ double sdot[DOT ? kMaxCols : 1][4];
+    const int per = (int)(gridDim.x >> 3);
+    const int blk = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    if (blk >= nblk) return;  // uniform per workgroup (grid rounded up to 8)
+    const int tid = threadIdx.x, r0 = blk * kSpmmRows, r = r0 + tid;
+    const int nown = min(kSpmmRows, n - r0);
+    const int h0 = hptr[blk], nh = hptr[blk + 1] - h0;
+    for (int j = tid; j < nh; j += kSpmmRows) s_h[j] = hcol[h0 + j];
+    int s = 0, t = 0;
+    if (r < n) {
+        s = rowptr[r];
+        t = rowptr[r + 1];
+    }
+    const bool active = r < n && t - s <= long_thr;  // long rows: k_spmm_long
+    const int cnt = active ? t - s : 0;
+    int lc[kHaloMaxE];
+    T v[kHaloMaxE];
+#pragma unroll
+    for (int u = 0; u < kHaloMaxE; u++) {
+        lc[u] = u < cnt ? (int)lcol[s + u] : 0;
+        v[u] = u < cnt ? val[s + u] : T(0);
+    }
+    const int lane = tid & 63, wid = tid >> 6;
+    __syncthreads();
+    for (int c0 = 0; c0 < k; c0 += KC) {
+#pragma unroll
+        for (int cc = 0; cc < KC; cc++) {
+            const int c = min(c0 + cc, k - 1);
+            const T *xc = X + (g.use_idx ? (int64_t)g.idx[c] : (int64_t)c) * ldx;
+            T *sx = s_x + cc * hcap;
+            if (tid < nown) sx[tid] = xc[r0 + tid];
+            for (int j = tid; j < nh; j += kSpmmRows) sx[kSpmmRows + j] = xc[s_h[j]];
+        }
+        __syncthreads();
+        double d[KC];
+#pragma unroll
+        for (int cc = 0; cc < KC; cc++) {
+            const T *sx = s_x + cc * hcap;
+            T acc = T(0);
+#pragma unroll
+            for (int u = 0; u < kHaloMaxE; u++)
+                if (u < cnt) acc = acc + v[u] * sx[lc[u]];
+            for (int e = s + kHaloMaxE; e < s + cnt; e++) acc = acc + val[e] * sx[lcol[e]];
+            const int c = c0 + cc;
+            if (active && c < k) Y[(int64_t)c * ldy + r] = acc;
+            if (DOT) d[cc] = (active && c < k) ? (double)sx[tid] * (double)acc : 0.0;
+        }
+        if (DOT) {
+            const double colsum = wave_colsum<KC>(d, lane);
+            constexpr int L = KC == 1 ? 0 : KC == 2 ? 1 : KC == 4 ? 2 : KC == 8 ? 3 : KC == 16 ? 4 : 5;
+            const int c = c0 + (lane >> (6 - L));
+            if ((lane & ((64 >> L) - 1)) == 0 && c < k) sdot[c][wid] = colsum;
+        }
+        __syncthreads();  // the next chunk overwrites s_x
+    }
+    if (DOT) {
+        for (int c = tid; c < k; c += kSpmmRows)
+            partial[(int64_t)c * pstride + blk] = (sdot[c][0] + sdot[c][1]) + (sdot[c][2] + sdot[c][3]);
+    }
+}
+
 // ------------------------------------------------- SpMM, row-major gather
 // Matrices whose column indices scatter (random graphs) make the column-major
 // gather of k_spmm touch one cache line per stored entry and column, and
@@ -912,13 +1002,13 @@ __global__ __launch_bounds__(kBlock) void k_csr_gather_sum(int nparam, const int
                                                            const int *__restrict__ jnz,
                                                            const double *__restrict__ jval,
                                                            const double *__restrict__ g,
-                                                           double *__restrict__ grad)
+                                                           double *__restrict__ grad, double scale, int accumulate)
 {
     const int p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= nparam) return;
     double acc = 0.0;
     for (int t = jrp[p]; t < jrp[p + 1]; t++) acc = fma(jval[t], g[jnz[t]], acc);
-    grad[p] = acc;
+    grad[p] = accumulate ? fma(scale, acc, grad[p]) : scale * acc;
 }
 
 __global__ __launch_bounds__(kBlock) void k_rowidx(int n, const int *__restrict__ rowptr,
@@ -986,26 +1076,51 @@ static bool reduce_to_host(double *partial, int rows, int nb, double *out)
     return true;
 }
 
+int dots_blocks(int64_t n) { return grid_1d(n, 256); }
+
+static void launch_dots(int prec, int64_t n, const void *x, int64_t ldx, const void *y, int64_t ldy, int k,
+                        double *partial, int nb)
+{
+    dim3 grid(nb, k);
+    if (prec == GCGE_HIP_FP64)
+        k_dots<double><<<grid, kBlock, 0, ctx().stream>>>(n, (const double *)x, ldx, (const double *)y, ldy, partial);
+    else
+        k_dots<float><<<grid, kBlock, 0, ctx().stream>>>(n, (const float *)x, ldx, (const float *)y, ldy, partial);
+}
+
 bool dots(int prec, int64_t n, const void *x, int64_t ldx, const void *y, int64_t ldy, int k,
           double *out)
 {
     if (k <= 0) return true;
     if (failed()) return false;
-    const int nb = grid_1d(n, 256);
+    const int nb = dots_blocks(n);
     double *partial = (double *)scratch(sizeof(double) * ((size_t)k * nb + k));
     if (!partial) return false;
-    dim3 grid(nb, k);
     {
         ProfScope ps(K_DOTS, 2.0 * k * (double)n * elem_size(prec), 2.0 * k * (double)n);
-        if (prec == GCGE_HIP_FP64)
-            k_dots<double><<<grid, kBlock, 0, ctx().stream>>>(n, (const double *)x, ldx, (const double *)y, ldy,
-                                                             partial);
-        else
-            k_dots<float><<<grid, kBlock, 0, ctx().stream>>>(n, (const float *)x, ldx, (const float *)y, ldy,
-                                                            partial);
+        launch_dots(prec, n, x, ldx, y, ldy, k, partial, nb);
     }
     if (!check(hipGetLastError(), "k_dots")) return false;
     return reduce_to_host(partial, k, nb, out);
+}
+
+// The same dots with the partial sums in a caller-provided device buffer of
+// at least k * dots_blocks(n) + k doubles and the sums copied on the backend
+// stream into host memory out (pinned or not); no allocation, no profiler.
+bool dots_with(int prec, int64_t n, const void *x, int64_t ldx, const void *y, int64_t ldy, int k, double *partial,
+               double *out)
+{
+    if (k <= 0) return true;
+    if (failed()) return false;
+    const int nb = dots_blocks(n);
+    launch_dots(prec, n, x, ldx, y, ldy, k, partial, nb);
+    if (!check(hipGetLastError(), "k_dots")) return false;
+    double *dres = partial + (int64_t)k * nb;
+    k_reduce_rows<<<k, kBlock, 0, ctx().stream>>>(partial, nb, dres);
+    if (!check(hipGetLastError(), "k_reduce_rows")) return false;
+    if (!check(hipMemcpyAsync(out, dres, sizeof(double) * k, hipMemcpyDeviceToHost, ctx().stream), "D2H dots"))
+        return false;
+    return check(hipStreamSynchronize(ctx().stream), "hipStreamSynchronize");
 }
 
 void launch_copy(int xprec, const void *x, int64_t ldx, int yprec, void *y, int64_t ldy, int64_t n, int k)
@@ -1098,6 +1213,64 @@ static bool spmm_rows(HIP_MAT *A, const T *val, const T *Xc, int64_t ldx, ColSca
     return spmm_rows_g<T, DOT, 2>(A, val, Xc, ldx, g, Yc, ldy, kc, out);
 }
 
+// Columns per LDS pass of k_spmm_halo: the largest power of two up to 8
+// whose staged rows fit the byte budget (GCGE_HIP_HALO_LDS, default 40 KiB),
+// or GCGE_HIP_HALO_KC.
+static int halo_kc(const HIP_MAT *A, size_t elem)
+{
+    const char *e = getenv("GCGE_HIP_HALO_KC");
+    const int env_kc = e ? atoi(e) : 0;
+    const char *l = getenv("GCGE_HIP_HALO_LDS");
+    const long env_lds = l ? atol(l) : 40960;
+    int kc = (env_kc == 1 || env_kc == 2 || env_kc == 4 || env_kc == 8) ? env_kc : 8;
+    if (env_kc != kc)
+        while (kc > 1 && (size_t)kc * A->halo_cap * elem > (size_t)env_lds) kc /= 2;
+    // never beyond 56 KiB of dynamic LDS (the kernel adds 4 KiB of its own)
+    while (kc > 1 && (size_t)kc * A->halo_cap * elem + (size_t)A->halo_cap * 4 > (size_t)56 * 1024) kc /= 2;
+    return kc;
+}
+
+template <typename T, bool DOT, int KC>
+static void spmm_halo_kc(HIP_MAT *A, const T *val, const T *Xc, int64_t ldx, ColScalars g, T *Yc, int64_t ldy,
+                         int kc, double *partial, int nblk, int long_thr, int pstride)
+{
+    const size_t lds = (size_t)KC * A->halo_cap * sizeof(T) + (size_t)(A->halo_cap - kSpmmRows) * sizeof(int);
+    k_spmm_halo<T, DOT, KC><<<((nblk + 7) / 8) * 8, kSpmmRows, lds, ctx().stream>>>(
+        A->n, A->rowptr, A->lcol, val, A->halo_ptr, A->halo_col, Xc, ldx, Yc, ldy, kc, g, partial, nblk, long_thr,
+        pstride, A->halo_cap);
+}
+
+template <typename T, bool DOT>
+static bool spmm_halo(HIP_MAT *A, const T *val, const T *Xc, int64_t ldx, ColScalars g, T *Yc, int64_t ldy,
+                      int kc, double *out)
+{
+    const int nblk = (A->n + kSpmmRows - 1) / kSpmmRows;
+    const int nlong = A->nlong, long_thr = nlong > 0 ? kLongRow : INT_MAX;
+    const int pstride = nblk + (DOT ? nlong : 0);
+    double *partial = nullptr;
+    if (DOT) {
+        partial = (double *)scratch(sizeof(double) * ((size_t)kc * pstride + kc));
+        if (!partial) return false;
+    }
+    {
+        const double bytes = (sizeof(T) + 4.0) * A->nnz + 4.0 * (A->n + 1) + 2.0 * kc * (double)A->n * sizeof(T);
+        const bool f64 = sizeof(T) == 8;
+        ProfScope ps(DOT ? (f64 ? K_SPMM_DOT_F64 : K_SPMM_DOT_F32) : (f64 ? K_SPMM_F64 : K_SPMM_F32), bytes,
+                     2.0 * kc * (double)A->nnz);
+        switch (halo_kc(A, sizeof(T))) {
+        case 8: spmm_halo_kc<T, DOT, 8>(A, val, Xc, ldx, g, Yc, ldy, kc, partial, nblk, long_thr, pstride); break;
+        case 4: spmm_halo_kc<T, DOT, 4>(A, val, Xc, ldx, g, Yc, ldy, kc, partial, nblk, long_thr, pstride); break;
+        case 2: spmm_halo_kc<T, DOT, 2>(A, val, Xc, ldx, g, Yc, ldy, kc, partial, nblk, long_thr, pstride); break;
+        default: spmm_halo_kc<T, DOT, 1>(A, val, Xc, ldx, g, Yc, ldy, kc, partial, nblk, long_thr, pstride); break;
+        }
+        if (nlong > 0)
+            k_spmm_long<T, DOT><<<(nlong + kSpmmRows / 64 - 1) / (kSpmmRows / 64), kSpmmRows, 0, ctx().stream>>>(
+                A->rowptr, A->col, val, Xc, ldx, Yc, ldy, kc, g, A->long_rows, nlong, partial, pstride, nblk);
+    }
+    if (!check(hipGetLastError(), "k_spmm_halo")) return false;
+    return !DOT || reduce_to_host(partial, kc, pstride, out);
+}
+
 template <typename T, bool DOT>
 static bool spmm_t(HIP_MAT *A, const void *X, int64_t ldx, void *Y, int64_t ldy, int k, const int *xidx,
                    double *out)
@@ -1112,6 +1285,10 @@ static bool spmm_t(HIP_MAT *A, const void *X, int64_t ldx, void *Y, int64_t ldy,
         T *Yc = (T *)Y + (int64_t)c0 * ldy;
         if (A->gather_rows && kc >= 2) {  // one column gathers a line per entry either way
             if (!spmm_rows<T, DOT>(A, val, Xc, ldx, g, Yc, ldy, kc, DOT ? out + c0 : nullptr)) return false;
+            continue;
+        }
+        if (A->halo) {
+            if (!spmm_halo<T, DOT>(A, val, Xc, ldx, g, Yc, ldy, kc, DOT ? out + c0 : nullptr)) return false;
             continue;
         }
         // long rows (power-law graphs) go to k_spmm_long; their dot partials
@@ -1425,11 +1602,11 @@ bool sampled_outer(int prec, HIP_MAT *A, const void *X, int64_t ldx, int k, cons
 }
 
 void launch_csr_gather_sum(int nparam, const int *jrowptr, const int *jnz, const double *jval, const double *g,
-                           double *grad)
+                           double *grad, double scale, bool accumulate)
 {
     if (nparam <= 0 || failed()) return;
     k_csr_gather_sum<<<(nparam + kBlock - 1) / kBlock, kBlock, 0, ctx().stream>>>(nparam, jrowptr, jnz, jval, g,
-                                                                                   grad);
+                                                                                   grad, scale, accumulate ? 1 : 0);
     check(hipGetLastError(), "k_csr_gather_sum");
 }
 

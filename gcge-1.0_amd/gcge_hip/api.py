@@ -56,6 +56,20 @@ class Matrix:
             _check("HIP_MatCreateFromCSR")
             raise RuntimeError("HIP_MatCreateFromCSR failed")
 
+    @classmethod
+    def from_file(cls, path):
+        """HIP_ReadMatFile: the reference's CSR text format or the binary form."""
+        L = capi.load()
+        ptr = L.HIP_ReadMatFile(str(path).encode())
+        if not ptr:
+            _check("HIP_ReadMatFile")
+            raise RuntimeError("HIP_ReadMatFile failed")
+        self = cls.__new__(cls)
+        self.ptr = ptr
+        self.n = L.HIP_MatNumRows(ptr)
+        self.nnz = L.HIP_MatNumNonzeros(ptr)
+        return self
+
     def set_values(self, val):
         v = np.ascontiguousarray(val, dtype=np.float64)
         capi.load().HIP_MatSetValues(self.ptr, _dp(v))
@@ -120,6 +134,16 @@ class MultiVec:
             self.close()
         except Exception:
             pass
+
+
+def write_matrix_file(path, M, binary=False):
+    """HIP_WriteMatFile of a scipy CSR matrix (text: the reference's format)."""
+    rp = np.ascontiguousarray(M.indptr, dtype=np.int32)
+    ci = np.ascontiguousarray(M.indices, dtype=np.int32)
+    v = np.ascontiguousarray(M.data, dtype=np.float64)
+    capi.load().HIP_WriteMatFile(str(path).encode(), M.shape[0], M.shape[1], int(ci.size), _ip(rp), _ip(ci), _dp(v),
+                                 1 if binary else 0)
+    _check("HIP_WriteMatFile")
 
 
 def _r(a, b):
@@ -289,6 +313,149 @@ FP32_REFINE_DEFAULTS = dict(MIXED_DEFAULTS, fp32_w_orth=0, refine_w_orth=0, refi
                             bcg_precond=0, rr_eig_full=0)
 
 
+class Solver:
+    """A set-up GCGE solver on the device, reusable across solves.
+
+    GCGE_HIP_Solver_Init runs once (workspace allocation included); each
+    ``solve(X0)`` writes the initial block and calls GCGE_SOLVER_Solve or
+    GCGE_HIP_SOLVER_SolveMixed, which restart the per-solve state and reuse
+    the workspace (and, for the two-phase forms, the FP32 phase's solver).
+    ``given_x0`` says whether the solves receive an initial block
+    (-gcge_given_init_evec 1) or draw it with rand()."""
+
+    def __init__(self, A, B, nev, precision="fp64", mixed=None, seed=None, given_x0=True, **params):
+        L = capi.load()
+        self._own = []
+        self.mA = A if isinstance(A, Matrix) else Matrix(A)
+        if not isinstance(A, Matrix):
+            self._own.append(self.mA)
+        self.mB = None if B is None else (B if isinstance(B, Matrix) else Matrix(B))
+        if B is not None and not isinstance(B, Matrix):
+            self._own.append(self.mB)
+        if precision not in ("fp64", "fp64-cg32", "mixed", "fp32+refine"):
+            raise ValueError(precision)
+        self.precision = precision
+        self.nev = nev
+        self.n = self.mA.n
+        self.params = params
+        self.seed = seed
+        args = [b"gcge"]
+        for k, v in params.items():
+            if k in _CLI:
+                args += [("-gcge_" + k).encode(), str(v).encode()]
+        if given_x0:
+            args += [b"-gcge_given_init_evec", b"1"]
+        args += [b"-gcge_print_para", str(params.get("print_para", 0)).encode(),
+                 b"-gcge_print_result", str(params.get("print_result", 0)).encode()]
+        self._argv = (C.c_char_p * len(args))(*args)
+        if seed is not None:
+            self._srand(seed)
+        self.solver = L.GCGE_HIP_Solver_Init(self.mA.ptr, self.mB.ptr if self.mB else None, nev, len(args),
+                                             self._argv)
+        if not self.solver:
+            _check("GCGE_HIP_Solver_Init")
+            raise RuntimeError("GCGE_HIP_Solver_Init failed")
+        sp = self.solver.contents
+        if sp.workspace.contents.V is None:
+            self.close()
+            raise RuntimeError("solver setup failed (missing operations)")
+        para = sp.para.contents
+        for k in _DIRECT:
+            if k in params:
+                setattr(para, k, params[k])
+        if precision == "fp64-cg32":
+            para.bcg_precision = 1
+        self.mixed_opts = None
+        if precision in ("mixed", "fp32+refine"):
+            o = capi.GCGE_HIP_MIXED_OPTS()
+            opts = dict(MIXED_DEFAULTS if precision == "mixed" else FP32_REFINE_DEFAULTS)
+            opts.update(mixed or {})
+            for f, _ in capi.GCGE_HIP_MIXED_OPTS._fields_:
+                setattr(o, f, opts[f])
+            self.mixed_opts = o
+
+    @staticmethod
+    def _srand(seed):
+        import ctypes.util
+        libc = C.CDLL(ctypes.util.find_library("c"))
+        libc.srand(C.c_uint(seed))
+
+    def set_x0(self, X0):
+        """Write the initial block: an (n, nev) host array or an FP64 MultiVec in HBM."""
+        L = capi.load()
+        sp = self.solver.contents
+        if isinstance(X0, MultiVec):
+            # a block already in HBM: device-to-device copy into evec
+            s, e = _r((0, 0), (self.nev, self.nev))
+            fn = C.cast(sp.ops.contents.MultiVecCopy, capi.F_MultiVecCopy)
+            fn(X0.ptr, sp.evec, s, e, sp.ops)
+        else:
+            X0 = np.asfortranarray(X0, dtype=np.float64)
+            L.HIP_MultiVecSetFromHost(sp.evec, 0, self.nev, _dp(X0), self.n)
+        _check("initial block")
+
+    def solve(self, X0=None, keep_evec=True):
+        L = capi.load()
+        sp = self.solver.contents
+        para = sp.para.contents
+        nev, n = self.nev, self.n
+        if X0 is not None:
+            self.set_x0(X0)
+        info = {}
+        rc = 0
+        t0 = time.perf_counter()
+        if self.mixed_opts is None:
+            L.GCGE_SOLVER_Solve(self.solver)
+        else:
+            mi = capi.GCGE_HIP_MIXED_INFO()
+            rc = L.GCGE_HIP_SOLVER_SolveMixed(self.solver, C.byref(self.mixed_opts), C.byref(mi))
+            info = {f: getattr(mi, f) for f, _ in capi.GCGE_HIP_MIXED_INFO._fields_}
+        L.GCGE_HIP_Synchronize()
+        wall = time.perf_counter() - t0
+        _check("solve")
+        ev = np.array(sp.eval[:nev])
+        res = np.array(para.res[:nev])
+        X = None
+        if keep_evec:
+            X = np.zeros((n, nev), order="F")
+            L.HIP_MultiVecGetToHost(sp.evec, 0, nev, _dp(X), n)
+            _check("download evec")
+        out = dict(eval=ev, evec=X, res=res, num_iter=para.num_iter, num_unlock=para.num_unlock,
+                   seconds=wall if self.mixed_opts is not None else para.solve_seconds, wall=wall,
+                   bcg_calls=para.bcg_calls, bcg_iters_total=para.bcg_iters_total, info=info, rc=rc)
+        pt = para.stat_para.contents.part_time_total.contents
+        out["part_time"] = {f: getattr(pt, f) for f, _ in capi.GCGE_PART_TIME._fields_}
+        return out
+
+    def residual_norms(self):
+        """||A x_i - lambda_i B x_i||, ||x_i|| and ||B x_i|| of the solver's
+        current eigenpairs, in FP64 on the device (MultiVecResidualNorms)."""
+        sp = self.solver.contents
+        nev = self.nev
+        idx = np.arange(nev, dtype=np.int32)
+        lam = np.array(sp.eval[:nev])
+        rn, xn, bn = np.zeros(nev), np.zeros(nev), np.zeros(nev)
+        fn = C.cast(sp.ops.contents.MultiVecResidualNorms, capi.F_MultiVecResidualNorms)
+        fn(self.mA.ptr, self.mB.ptr if self.mB else None, sp.evec, _ip(idx), nev, _dp(lam), _dp(rn), _dp(xn),
+           _dp(bn), sp.ops)
+        _check("MultiVecResidualNorms")
+        return rn, xn, bn
+
+    def close(self):
+        if getattr(self, "solver", None):
+            capi.load().GCGE_SOLVER_Free_All(C.byref(self.solver))
+            self.solver = None
+        for m in getattr(self, "_own", []):
+            m.close()
+        self._own = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def solve(A, B, nev, X0=None, precision="fp64", mixed=None, seed=None, keep_evec=True, **params):
     """Solve A x = lambda B x on the device.
 
@@ -309,90 +476,39 @@ def solve(A, B, nev, X0=None, precision="fp64", mixed=None, seed=None, keep_evec
     X0 is an (n, nev) host array or an FP64 MultiVec already on the device.
     ``mixed`` overrides GCGE_HIP_MIXED_OPTS fields.  Returns a dict with
     eval, evec, res, num_iter, num_unlock, seconds (GCGE_Solve wall time, or
-    the whole two-phase solve), bcg statistics and, for the two-phase forms,
-    the phase info."""
-    L = capi.load()
-    mA = A if isinstance(A, Matrix) else Matrix(A)
-    mB = None if B is None else (B if isinstance(B, Matrix) else Matrix(B))
-    args = [b"gcge"]
-    keep = []
-    for k, v in params.items():
-        if k in _CLI:
-            args += [("-gcge_" + k).encode(), str(v).encode()]
-    if X0 is not None:
-        args += [b"-gcge_given_init_evec", b"1"]
-    args += [b"-gcge_print_para", str(params.get("print_para", 0)).encode(),
-             b"-gcge_print_result", str(params.get("print_result", 0)).encode()]
-    argv = (C.c_char_p * len(args))(*args)
-    keep.append(argv)
-    if seed is not None:
-        import ctypes.util
-        libc = C.CDLL(ctypes.util.find_library("c"))
-        libc.srand(C.c_uint(seed))
-    solver = L.GCGE_HIP_Solver_Init(mA.ptr, mB.ptr if mB else None, nev, len(args), argv)
-    if not solver:
-        _check("GCGE_HIP_Solver_Init")
-        raise RuntimeError("GCGE_HIP_Solver_Init failed")
-    sp = solver.contents
-    para = sp.para.contents
-    for k in _DIRECT:
-        if k in params:
-            setattr(para, k, params[k])
-    n = mA.n
+    the whole two-phase solve), bcg statistics, the return code ``rc`` of the
+    two-phase call and, for the two-phase forms, the phase info."""
+    s = Solver(A, B, nev, precision=precision, mixed=mixed, seed=seed, given_x0=X0 is not None, **params)
     try:
-        if sp.workspace.contents.V is None:
-            raise RuntimeError("solver setup failed (missing operations)")
-        if isinstance(X0, MultiVec):
-            # a block already in HBM: device-to-device copy into evec
-            s, e = _r((0, 0), (nev, nev))
-            fn = C.cast(sp.ops.contents.MultiVecCopy, capi.F_MultiVecCopy)
-            fn(X0.ptr, sp.evec, s, e, sp.ops)
-            _check("initial block")
-        elif X0 is not None:
-            X0 = np.asfortranarray(X0, dtype=np.float64)
-            L.HIP_MultiVecSetFromHost(sp.evec, 0, nev, _dp(X0), n)
-            _check("initial block")
-        info = {}
-        t0 = time.perf_counter()
-        if precision == "fp64":
-            L.GCGE_SOLVER_Solve(solver)
-        elif precision == "fp64-cg32":
-            para.bcg_precision = 1
-            L.GCGE_SOLVER_Solve(solver)
-        elif precision in ("mixed", "fp32+refine"):
-            o = capi.GCGE_HIP_MIXED_OPTS()
-            opts = dict(MIXED_DEFAULTS if precision == "mixed" else FP32_REFINE_DEFAULTS)
-            opts.update(mixed or {})
-            for f, _ in capi.GCGE_HIP_MIXED_OPTS._fields_:
-                setattr(o, f, opts[f])
-            mi = capi.GCGE_HIP_MIXED_INFO()
-            L.GCGE_HIP_SOLVER_SolveMixed(solver, C.byref(o), C.byref(mi))
-            info = dict(fp32_iters=mi.fp32_iters, fp64_iters=mi.fp64_iters, fp32_seconds=mi.fp32_seconds,
-                        fp64_seconds=mi.fp64_seconds, fp64_num_unlock=mi.fp64_num_unlock)
-        else:
-            raise ValueError(precision)
-        L.GCGE_HIP_Synchronize()
-        wall = time.perf_counter() - t0
-        _check("solve")
-        ev = np.array(sp.eval[:nev])
-        res = np.array(para.res[:nev])
-        X = None
-        if keep_evec:
-            X = np.zeros((n, nev), order="F")
-            L.HIP_MultiVecGetToHost(sp.evec, 0, nev, _dp(X), n)
-            _check("download evec")
-        out = dict(eval=ev, evec=X, res=res, num_iter=para.num_iter, num_unlock=para.num_unlock,
-                   seconds=wall if precision in ("mixed", "fp32+refine") else para.solve_seconds, wall=wall,
-                   bcg_calls=para.bcg_calls, bcg_iters_total=para.bcg_iters_total, info=info)
-        pt = para.stat_para.contents.part_time_total.contents
-        out["part_time"] = {f: getattr(pt, f) for f, _ in capi.GCGE_PART_TIME._fields_}
-        return out
+        return s.solve(X0, keep_evec=keep_evec)
     finally:
-        L.GCGE_SOLVER_Free_All(C.byref(solver))
-        if not isinstance(A, Matrix):
-            mA.close()
-        if mB is not None and not isinstance(B, Matrix):
-            mB.close()
+        s.close()
+
+
+def eig_gradient_generalized(A: Matrix, B, X, lam, weights, nparam, jA, jB=None):
+    """d(sum_i w_i lambda_i)/d theta for A(theta) x = lambda B(theta) x via
+    GCGE_HIP_EigGradientGeneralized; X holds B-normalized eigenvectors, jA
+    and jB are (jrowptr, jnz, jval) Jacobians of A's and B's stored values
+    (jB None: B does not depend on theta)."""
+    L = capi.load()
+    k = X.shape[1]
+    mv = MultiVec(A, k, data=X)
+    w = np.ascontiguousarray(weights, dtype=np.float64)
+    lam = np.ascontiguousarray(lam, dtype=np.float64)
+
+    def arrs(j):
+        return (np.ascontiguousarray(j[0], dtype=np.int32), np.ascontiguousarray(j[1], dtype=np.int32),
+                np.ascontiguousarray(j[2], dtype=np.float64))
+
+    a = arrs(jA)
+    b = arrs(jB) if jB is not None else None
+    grad = np.zeros(nparam)
+    L.GCGE_HIP_EigGradientGeneralized(A.ptr, B.ptr if B is not None else None, mv.ptr, k, _dp(lam), _dp(w), nparam,
+                                      _ip(a[0]), _ip(a[1]), _dp(a[2]), _ip(b[0]) if b else None,
+                                      _ip(b[1]) if b else None, _dp(b[2]) if b else None, _dp(grad))
+    _check("GCGE_HIP_EigGradientGeneralized")
+    mv.close()
+    return grad
 
 
 def eig_gradient(A: Matrix, X, weights, nparam, jrowptr, jnz, jval):

@@ -80,6 +80,7 @@ class GCGE_WORKSPACE(C.Structure):
         ("num_soft_locked_in_last_iter", c_int), ("dense_matrix", c_void_p),
         ("subspace_matrix", DP), ("subspace_evec", DP), ("subspace_dtmp", DP), ("subspace_itmp", IP),
         ("eval", DP), ("orth_ind", IP), ("bcg32", c_void_p * 4), ("bcg32_cols", c_int),
+        ("mixed32", c_void_p),
     ]
 
 
@@ -119,7 +120,10 @@ class GCGE_HIP_MIXED_OPTS(C.Structure):
 
 class GCGE_HIP_MIXED_INFO(C.Structure):
     _fields_ = [("fp32_iters", c_int), ("fp64_iters", c_int), ("fp32_seconds", c_double),
-                ("fp64_seconds", c_double), ("fp64_num_unlock", c_int)]
+                ("fp64_seconds", c_double), ("fp64_num_unlock", c_int), ("refine_passes", c_int),
+                ("fp32_failed", c_int)]
+
+GCGE_HIP_NOT_CONVERGED = -2
 
 
 SP = C.POINTER(GCGE_SOLVER)
@@ -163,6 +167,7 @@ SIGNATURES = {
     "GCGE_CheckEvalMultiplicity": (None, [c_int, c_int, c_int, IP, DP]),
     "HIP_MatCreateFromCSR": (c_void_p, [c_int, c_int, c_int, IP, IP, DP]),
     "HIP_ReadMatFile": (c_void_p, [c_char_p]),
+    "HIP_WriteMatFile": (c_int, [c_char_p, c_int, c_int, c_int, IP, IP, DP, c_int]),
     "HIP_MatDestroy": (None, [C.POINTER(c_void_p)]),
     "HIP_MatNumRows": (c_int, [c_void_p]),
     "HIP_MatNumNonzeros": (c_int, [c_void_p]),
@@ -179,6 +184,8 @@ SIGNATURES = {
     "GCGE_HIP_Solver_Init": (SP, [c_void_p, c_void_p, c_int, c_int, C.POINTER(c_char_p)]),
     "GCGE_HIP_SOLVER_SolveMixed": (c_int, [SP, C.POINTER(GCGE_HIP_MIXED_OPTS), C.POINTER(GCGE_HIP_MIXED_INFO)]),
     "GCGE_HIP_EigGradient": (c_int, [c_void_p, c_void_p, c_int, DP, c_int, IP, IP, DP, DP]),
+    "GCGE_HIP_EigGradientGeneralized": (c_int, [c_void_p, c_void_p, c_void_p, c_int, DP, DP, c_int, IP, IP, DP, IP,
+                                                IP, DP, DP]),
     "GCGE_HIP_GetLastError": (c_int, [c_char_p, c_int]),
     "GCGE_HIP_ClearError": (None, []),
     "GCGE_HIP_Synchronize": (c_int, []),

@@ -35,13 +35,23 @@ enum { GCGE_HIP_FP64 = 0, GCGE_HIP_FP32 = 1 };
  * FP64 val[nnz].  Columns need not be sorted.  Returns NULL on error. */
 HIP_MAT *HIP_MatCreateFromCSR(GCGE_INT nrows, GCGE_INT ncols, GCGE_INT nnz, const GCGE_INT *rowptr,
                               const GCGE_INT *col, const GCGE_DOUBLE *val);
-/* Read the text format of CSR_ReadMatFile (app/csr/gcge_app_csr.c:185-236):
- * nrow, ncol, nnz, then rowptr, col, val, whitespace separated. */
+/* Read a matrix file and upload it (HIP_MatCreateFromCSR).  Two formats:
+ * the text format of CSR_ReadMatFile (app/csr/gcge_app_csr.c:185-236):
+ * nrow, ncol, nnz, then rowptr, col, val, whitespace separated; and its
+ * binary fast path, recognized by the magic "GCGECSR1": int32 nrow, ncol,
+ * nnz, 0, then rowptr[nrow+1], col[nnz] (int32) and val[nnz] (FP64), little
+ * endian, as HIP_WriteMatFile writes it. */
 HIP_MAT *HIP_ReadMatFile(const char *filename);
+/* Write CSR arrays (host) in the text format (binary = 0; values with 17
+ * significant digits, so they read back exactly) or the binary one. */
+GCGE_INT HIP_WriteMatFile(const char *filename, GCGE_INT nrows, GCGE_INT ncols, GCGE_INT nnz,
+                          const GCGE_INT *rowptr, const GCGE_INT *col, const GCGE_DOUBLE *val,
+                          GCGE_INT binary);
 void HIP_MatDestroy(HIP_MAT **mat);
 GCGE_INT HIP_MatNumRows(const HIP_MAT *mat);
 GCGE_INT HIP_MatNumNonzeros(const HIP_MAT *mat);
-/* Replace the values (same sparsity pattern). */
+/* Replace the values (same sparsity pattern); the FP32 copy, the Jacobi
+ * diagonal and the Gershgorin bound are refreshed with them. */
 GCGE_INT HIP_MatSetValues(HIP_MAT *mat, const GCGE_DOUBLE *val);
 /* Storage precision of the multivectors and vectors created from this matrix
  * from now on (GCGE_HIP_FP64 or GCGE_HIP_FP32).  Matrix products with FP32
@@ -100,8 +110,16 @@ typedef struct {
     GCGE_INT fp64_iters;
     GCGE_DOUBLE fp32_seconds;
     GCGE_DOUBLE fp64_seconds;
-    GCGE_INT fp64_num_unlock;
+    GCGE_INT fp64_num_unlock; /* pairs above the tolerance at the final check of every pair */
+    GCGE_INT refine_passes;   /* FP64 refinement passes run (restarts + 1)                 */
+    GCGE_INT fp32_failed;     /* 0: the FP32 phase started the refinement; 1: it ended with
+                                 a non-finite residual and was discarded; 2: skipped, A or B
+                                 out of FP32 range                                         */
 } GCGE_HIP_MIXED_INFO;
+
+/* Return value of GCGE_HIP_SOLVER_SolveMixed when no error is latched but
+ * pairs are still above the tolerance after the refinement budget. */
+#define GCGE_HIP_NOT_CONVERGED (-2)
 
 /* Solve with an FP32 phase (matrix values and multivectors in FP32, dense
  * Rayleigh-Ritz, Cholesky factors and the residual check in FP64) followed by
@@ -115,7 +133,14 @@ typedef struct {
  * with the diagonal of A and capped at 30 CG steps, and Rayleigh-Ritz pairs
  * from the full dsyevd decomposition.  In opts, a zero fp32_tol, budget or
  * stagnation_ratio selects the same default, and a zero cg_max_it,
- * bcg_precond or rr_eig_full keeps the caller's. */
+ * bcg_precond or rr_eig_full keeps the caller's.
+ * Every pair is checked under the caller's criterion at the end; para->res,
+ * para->max_res and para->num_unlock hold that check.  Returns 0 when every
+ * pair is within ev_tol, GCGE_HIP_NOT_CONVERGED when some are not, and the
+ * latched error code otherwise (a non-finite final residual latches 190).
+ * The FP32 phase is skipped when the Gershgorin bound of A or B lies outside
+ * [1e-18, 1e18], and discarded when it ends with a non-finite residual; the
+ * FP64 iteration then starts from the caller's block (or a random one). */
 GCGE_INT GCGE_HIP_SOLVER_SolveMixed(GCGE_SOLVER *solver, const GCGE_HIP_MIXED_OPTS *opts,
                                     GCGE_HIP_MIXED_INFO *info);
 
@@ -132,6 +157,21 @@ GCGE_INT GCGE_HIP_SOLVER_SolveMixed(GCGE_SOLVER *solver, const GCGE_HIP_MIXED_OP
 GCGE_INT GCGE_HIP_EigGradient(HIP_MAT *A, void **X, GCGE_INT k, const GCGE_DOUBLE *w,
                               GCGE_INT nparam, const GCGE_INT *jrowptr, const GCGE_INT *jnz,
                               const GCGE_DOUBLE *jval, GCGE_DOUBLE *grad);
+
+/* The generalized problem A(theta) x = lambda B(theta) x: with B-normalized
+ * eigenvectors X(:, i) (x_i^T B x_i = 1, as the solver returns them) and
+ * their eigenvalues lambda[i], writes
+ *   grad[p] = sum_i w_i x_i^T (dA/dtheta_p - lambda_i dB/dtheta_p) x_i,
+ * i.e. d(sum_i w_i lambda_i)/d theta.  The A and B Jacobians are given as
+ * for GCGE_HIP_EigGradient, each over its own matrix's stored entries and
+ * with the same nparam rows; B == NULL (B = I) or jb_rowptr == NULL (B does
+ * not depend on theta) drops the second term. */
+GCGE_INT GCGE_HIP_EigGradientGeneralized(HIP_MAT *A, HIP_MAT *B, void **X, GCGE_INT k,
+                                         const GCGE_DOUBLE *lambda, const GCGE_DOUBLE *w,
+                                         GCGE_INT nparam, const GCGE_INT *ja_rowptr,
+                                         const GCGE_INT *ja_nz, const GCGE_DOUBLE *ja_val,
+                                         const GCGE_INT *jb_rowptr, const GCGE_INT *jb_nz,
+                                         const GCGE_DOUBLE *jb_val, GCGE_DOUBLE *grad);
 
 /* ---- errors, synchronisation, counters ---- */
 

@@ -63,6 +63,8 @@ typedef struct GCGE_WORKSPACE_ {
     /* ---- members of this build (no counterpart in the reference) ---- */
     void **bcg32[4];     /* FP32 block-CG work columns (rhs/residual, solution, p, w) */
     GCGE_INT bcg32_cols; /* columns allocated in each of them                        */
+    void *mixed32;       /* solver of the FP32 phase of GCGE_HIP_SOLVER_SolveMixed,
+                            kept for later calls (NULL until the first)              */
 } GCGE_WORKSPACE;
 
 typedef struct GCGE_SOLVER_ {

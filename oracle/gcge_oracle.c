@@ -514,6 +514,180 @@ static void stable_multi_orth_insub(S_t *S, double *Vd, int ldV, int nrows, int 
 /* Rayleigh-Ritz (src/gcge_rayleighritz.c)                                   */
 /* ------------------------------------------------------------------------ */
 
+
+/* B*V[xs:xe] into Y[ys:...] (MatDotVec per column), or the copy
+ * VecAxpby(1.0, x, 0.0, y) when B is NULL. */
+static void b_times(S_t *S, const orc_csr *B, double **X, int xs, int xe, double **Y, int ys)
+{
+    int j;
+    for (j = xs; j < xe; j++) {
+        if (B == NULL) orc_vec_axpby(1.0, X[j], 0.0, Y[ys + j - xs], S->n);
+        else matvec(S, B, X[j], Y[ys + j - xs]);
+    }
+}
+
+/* MGS of V[start:end] against itself with zero-column deflation: the W-self
+ * loop of GCGE_BOrthonormalization (gcge_orthogonal.c:459-526), also
+ * GCGE_SCBOrth_Self's inner loop (:1205-1277). */
+static void w_self_mgs(S_t *S, double **V, int start, int *end, const orc_csr *B)
+{
+    int cur, j, ms[2], me[2];
+    double *d = S->subspace_dtmp, norm_value;
+    for (cur = start; cur < *end; cur++) {
+        b_times(S, B, V, cur, cur + 1, S->CG_p, 0);
+        ms[0] = cur; me[0] = *end; ms[1] = 0; me[1] = 1;
+        mv_innerprod(S, V, S->CG_p, d, "ns", ms, me, *end - cur);
+        norm_value = sqrt(d[0]);
+        if (norm_value > S->P.orth_zero_tol) {
+            orc_vec_axpby(0.0, V[cur], 1.0 / norm_value, V[cur], S->n);
+            for (j = cur + 1; j < *end; j++) orc_vec_axpby(-d[j - cur] / norm_value, V[cur], 1.0, V[j], S->n);
+        } else {
+            ms[0] = cur; ms[1] = *end - 1; me[0] = cur + 1; me[1] = *end;
+            mv_swap(V, V, ms, me);
+            (*end)--;
+            cur--;
+        }
+    }
+}
+
+/* GCGE_BOrthonormalization, gcge_orthogonal.c:352-528. */
+static void orth_bgs(S_t *S, double **V, int start, int *end, const orc_csr *B)
+{
+    int nev = S->P.nev, dim_xp = S->dim_xp, i, cur, j, wl, ms[2], me[2];
+    double *d = S->subspace_dtmp;
+    b_times(S, B, V, 0, nev, S->evec, 0);
+    b_times(S, B, V, nev, dim_xp, S->V_tmp, 0);
+    for (i = 0; i < 2; i++) {
+        wl = *end - start;
+        for (cur = 0; cur < dim_xp; cur++) {
+            double **BV = cur < nev ? S->evec : S->V_tmp;
+            int bc = cur < nev ? cur : cur - nev;
+            ms[0] = start; me[0] = *end; ms[1] = bc; me[1] = bc + 1;
+            mv_innerprod(S, V, BV, d, "ns", ms, me, wl);
+            for (j = start; j < *end; j++) orc_vec_axpby(-d[j - start], V[cur], 1.0, V[j], S->n);
+        }
+        w_self_mgs(S, V, start, end, B);
+    }
+}
+
+/* V_tmp = B W, d = V[p0:p1]^T V_tmp (leading dimension lda), V_tmp = V[p0:p1] d,
+ * W -= V_tmp, for W = V[start:end] (gcge_orthogonal.c:822-855); returns
+ * sqrt(sum d[0:nsum]^2). */
+static double project_out(S_t *S, double **V, int p0, int p1, int start, int end, int lda, int nsum,
+                          const orc_csr *B)
+{
+    int wl = end - start, ms[2], me[2], i;
+    double *d = S->subspace_dtmp, value_inner = 0.0, t;
+    ms[0] = start; me[0] = end; ms[1] = 0; me[1] = wl;
+    if (B == NULL) mv_axpby(S, 1.0, V, 0.0, S->V_tmp, ms, me);
+    else mv_matdot(S, B, V, S->V_tmp, ms, me);
+    ms[0] = p0; me[0] = p1; ms[1] = 0; me[1] = wl;
+    mv_innerprod(S, V, S->V_tmp, d, "ns", ms, me, lda);
+    for (i = 0; i < nsum; i++) {
+        t = d[i];
+        value_inner += t * t;
+    }
+    mv_lincomb(S, V, S->V_tmp, ms, me, d, lda);
+    ms[0] = 0; me[0] = wl; ms[1] = start; me[1] = end;
+    mv_axpby(S, -1.0, S->V_tmp, 1.0, V, ms, me);
+    return sqrt(value_inner);
+}
+
+/* GCGE_CBOrthonormalization, gcge_orthogonal.c:755-963. */
+static void orth_cbgs(S_t *S, double **V, int start, int *end, const orc_csr *B)
+{
+    int i, j, wl = *end - start, ms[2], me[2], il = 1, iu, m, lwork, info = 0, nzs;
+    double *d = S->subspace_dtmp, vl = 0.0, vu = 0.0, abstol = 0.0;
+    double *tmp_evec, *tmp_eval, *dwork;
+    int *ifail, nonzero = 0;
+    for (i = 0; i < 2; i++) {
+        wl = *end - start;
+        project_out(S, V, 0, start, start, *end, start, 0, B);
+    }
+    b_times(S, B, V, start, *end, S->V_tmp, 0);
+    ms[0] = start; me[0] = *end; ms[1] = 0; me[1] = wl;
+    mv_innerprod(S, V, S->V_tmp, d, "ns", ms, me, wl);
+    for (j = 0; j < wl * wl; j++)
+        if (d[j] > S->P.orth_zero_tol) {
+            nonzero = 1;
+            break;
+        }
+    if (!nonzero) {
+        *end = start;
+        return;
+    }
+    tmp_evec = S->subspace_evec;
+    tmp_eval = tmp_evec + (size_t)wl * wl;
+    dwork = d + (size_t)wl * wl;
+    ifail = S->subspace_itmp + 5 * wl;
+    iu = wl;
+    m = iu - il + 1;
+    lwork = 8 * wl;
+    dsyevx_("V", "I", "U", &wl, d, &wl, &vl, &vu, &il, &iu, &abstol, &m, tmp_eval, tmp_evec, &wl, dwork, &lwork,
+            S->subspace_itmp, ifail, &info);
+    nzs = 0;
+    for (j = 0; j < wl; j++) {
+        if (tmp_eval[j] < S->P.orth_zero_tol) {
+            nzs = j + 1;
+        } else {
+            double a = 1.0 / sqrt(tmp_eval[j]);
+            int len = wl, inc = 1;
+            dscal_(&len, &a, tmp_evec + (size_t)j * wl, &inc);
+        }
+    }
+    wl -= nzs;
+    /* the reference offsets and reads the eigenvectors with the new column
+     * count as the leading dimension (:936-945) */
+    ms[0] = start; me[0] = *end; ms[1] = 0; me[1] = wl;
+    mv_lincomb(S, V, S->V_tmp, ms, me, tmp_evec + (size_t)nzs * wl, wl);
+    *end = start + wl;
+    ms[0] = start; me[0] = *end; ms[1] = 0; me[1] = wl;
+    mv_swap(V, S->V_tmp, ms, me);
+}
+
+/* GCGE_SCBOrth_Minus, gcge_orthogonal.c:1116-1193. */
+static void scbgs_minus(S_t *S, double **V, int start, int end, const orc_csr *B)
+{
+    int wl = end - start, iter = 0, dim_xp = S->dim_xp;
+    double value_inner = 1.0;
+    while (value_inner > S->P.scbgs_reorth_tol) {
+        iter++;
+        value_inner = project_out(S, V, dim_xp, start, start, end, start, (start - dim_xp) * wl, B);
+        if (iter >= S->P.max_reorth_time) break;
+    }
+}
+
+/* GCGE_SCBOrthonormalization, gcge_orthogonal.c:967-1113, with
+ * GCGE_SCBOrth_Self (:1196-1279). */
+static void orth_scbgs(S_t *S, double **V, int start, int *end, const orc_csr *B)
+{
+    int wl = *end - start, iter = 0, step, w_start, w_end, old_w_end, it, ms[2], me[2];
+    double value_inner = 1.0;
+    while (value_inner > S->P.scbgs_reorth_tol) {
+        iter++;
+        value_inner = project_out(S, V, 0, start, start, *end, start, start * wl, B);
+        if (iter >= S->P.max_reorth_time) break;
+    }
+    step = (*end - start < S->P.w_orth_block_size) ? (*end - start) : S->P.w_orth_block_size;
+    w_start = start;
+    w_end = w_start + step;
+    if (w_end > *end) w_end = *end;
+    for (it = 0; it < S->P.scbgs_wself_max_reorth_time; it++) w_self_mgs(S, V, w_start, &w_end, B);
+    while (w_end < *end) {
+        w_start = w_end;
+        w_end = w_start + step;
+        if (w_end > *end) w_end = *end;
+        old_w_end = w_end;
+        scbgs_minus(S, V, w_start, w_end, B);
+        for (it = 0; it < S->P.scbgs_wself_max_reorth_time; it++) w_self_mgs(S, V, w_start, &w_end, B);
+        if (w_end < old_w_end) {
+            *end -= old_w_end - w_end;
+            ms[0] = w_end; ms[1] = *end; me[0] = old_w_end; me[1] = *end + old_w_end - w_end;
+            mv_swap(V, V, ms, me);
+        }
+    }
+}
+
 /* GCGE_ComputeSubspaceMatrixVTAW, gcge_rayleighritz.c:35-81. */
 static void rr_vtaw(S_t *S, double **V, const orc_csr *A, int start_W, int end_V, double *mat,
                     double **work)
@@ -548,15 +722,21 @@ static void rr_matrix(S_t *S, const orc_csr *A, double **V)
 }
 
 /* GCGE_ComputeSubspaceEigenpairs, gcge_rayleighritz.c:364-590 (single
- * process, opt_rr_eig_partly = 0, orth_type "B"). */
+ * process, orth_type "B").  With opt_rr_eig_partly the pairs before the
+ * first unconverged one (unlock[0]), less a cluster of equal eigenvalues
+ * reaching back from it (multi_tol_for_lock), are soft-locked and only the
+ * trailing block is decomposed (:380-394). */
 static int rr_eigenpairs(S_t *S)
 {
     int ldm = S->dim_xpw, info = 0, i;
     double *temp = S->subspace_dtmp, *dwork = temp + (size_t)ldm * ldm;
     double vl = 0.0, vu = 0.0, abstol = 0.0;
-    int rr_eigen_start = 0, il, iu, m, lwork = 8 * ldm, nrows, ncols;
+    int rr_eigen_start = S->unlock[0], il, iu, m, lwork = 8 * ldm, nrows, ncols;
     int *ifail = S->subspace_itmp + 5 * ldm;
+    if (S->P.opt_rr_eig_partly == 0) rr_eigen_start = 0;
     il = rr_eigen_start;
+    for (i = rr_eigen_start - 1; i > -1; i--)
+        if (fabs(S->eval[i] - S->eval[rr_eigen_start]) / S->eval[rr_eigen_start] < S->P.multi_tol_for_lock) il -= 1;
     rr_eigen_start = il;
     il += 1;
     S->num_soft_locked = rr_eigen_start;
@@ -858,6 +1038,8 @@ void orc_default_params(orc_params *p)
     p->dirichlet_boundary = 0;
     p->opt_rr_eig_partly = 0;
     p->seed = 1;
+    p->w_orth_block_size = 0;
+    p->scbgs_wself_max_reorth_time = 1;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -886,12 +1068,12 @@ int orc_solve(const orc_csr *A, const orc_csr *B, const orc_params *p, const dou
     nev = S->P.nev;
     if (nev <= 0 || nev > n) return -1;
     if (strcmp(S->P.orth_type, "B") != 0) return -2;          /* "A" not restated */
-    if (S->P.opt_rr_eig_partly != 0) return -3;               /* soft locking not restated */
     if (strcmp(S->P.p_orth_type, "multi") != 0) return -4;
 
     /* GCGE_PARA_Setup, gcge_para.c:357-386 */
     S->num_unlock = nev;
     if (S->P.block_size == 0) S->P.block_size = nev;
+    if (S->P.w_orth_block_size == 0) S->P.w_orth_block_size = (S->P.block_size / 5 > 1) ? S->P.block_size / 5 : 1;
     S->P.orth_zero_tol += DBL_EPSILON;
     S->P.scbgs_reorth_tol += DBL_EPSILON;
     S->res = (double *)calloc((size_t)nev, sizeof(double));
@@ -998,7 +1180,13 @@ int orc_solve(const orc_csr *A, const orc_csr *B, const orc_params *p, const dou
         compute_w(S, A, B, S->V, S->eval);
         S->last_dim_xpw = S->dim_xpw;
         S->dim_xpw = S->dim_xp + S->unconv_bs;
-        if (strcmp(S->P.w_orth_type, "multi") == 0)
+        if (strcmp(S->P.w_orth_type, "bgs") == 0)
+            orth_bgs(S, S->V, S->dim_xp, &S->dim_xpw, Orth_mat);
+        else if (strcmp(S->P.w_orth_type, "cbgs") == 0)
+            orth_cbgs(S, S->V, S->dim_xp, &S->dim_xpw, Orth_mat);
+        else if (strcmp(S->P.w_orth_type, "scbgs") == 0)
+            orth_scbgs(S, S->V, S->dim_xp, &S->dim_xpw, Orth_mat);
+        else if (strcmp(S->P.w_orth_type, "multi") == 0)
             stable_multi_orth(S, S->V, S->dim_xp, &S->dim_xpw, Orth_mat);
         else
             orth_gs(S, S->V, S->dim_xp, &S->dim_xpw, Orth_mat, S->V_tmp, S->subspace_dtmp);
@@ -1011,6 +1199,9 @@ int orc_solve(const orc_csr *A, const orc_csr *B, const orc_params *p, const dou
         print_iteration(S);
     }
 
+    /* soft-locked vectors live in V (gcge_eigsol.c:533-537) */
+    ms[0] = 0; me[0] = S->num_soft_locked; ms[1] = 0; me[1] = S->num_soft_locked;
+    if (me[0] > 0) mv_axpby(S, 1.0, S->V, 0.0, S->evec, ms, me);
     memcpy(eval_out, S->eval, (size_t)nev * sizeof(double));
     if (evec_out)
         for (i = 0; i < nev; i++)

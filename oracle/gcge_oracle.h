@@ -43,7 +43,7 @@ typedef struct {
     double ev_tol;
     const char *conv_type;      /* "R", "O", "A", or the extension "NS"      */
     const char *orth_type;      /* "B" (only B-orthogonalization is restated) */
-    const char *w_orth_type;    /* "multi" or "gs"                           */
+    const char *w_orth_type;    /* "multi", "gs", "bgs", "cbgs" or "scbgs"   */
     const char *p_orth_type;    /* "multi"                                   */
     double conv_omega_norm;
     double orth_zero_tol;       /* DBL_EPSILON is added in setup             */
@@ -60,6 +60,8 @@ typedef struct {
     int    dirichlet_boundary;
     int    opt_rr_eig_partly;
     unsigned seed;              /* srand(seed) before rand() initialisation  */
+    int    w_orth_block_size;   /* scbgs W block; 0 -> max(block_size/5, 1)  */
+    int    scbgs_wself_max_reorth_time;
 } orc_params;
 
 typedef struct {

@@ -46,7 +46,8 @@ class OrcParams(C.Structure):
         ("max_direct_orth_length", C.c_int), ("multi_tol_for_lock", C.c_double),
         ("cg_max_it", C.c_int), ("cg_rate", C.c_double), ("cg_type", C.c_int),
         ("if_use_bcg", C.c_int), ("dirichlet_boundary", C.c_int),
-        ("opt_rr_eig_partly", C.c_int), ("seed", C.c_uint),
+        ("opt_rr_eig_partly", C.c_int), ("seed", C.c_uint), ("w_orth_block_size", C.c_int),
+        ("scbgs_wself_max_reorth_time", C.c_int),
     ]
 
 

@@ -8,8 +8,8 @@ counts and the per-iteration eigenvalue/residual history it prints.  The CSR
 matrices A_3/M_3, A_5/M_5 and testA/testB are the data files of the
 reference's own tests (test/data/), stored here as arrays.
 
-Run (in a container that has /root/reference):
-    MKL_NUM_THREADS=1 python tests/golden/make_golden.py
+Run (in a container that has /root/reference; names select fixtures):
+    MKL_NUM_THREADS=1 python tests/golden/make_golden.py [name ...]
 """
 from __future__ import annotations
 
@@ -81,6 +81,26 @@ def cases():
         # 3x3 SPD matrix of test/data/testA with B = I3 from testB
         ("testA_nev1", tA, tB, 1, None, 3, 0, [], {}),
     ]
+    # the other W orthonormalizations of the reference (gcge_eigsol.c:448-463),
+    # standard and generalized
+    for wt in ("bgs", "cbgs", "scbgs"):
+        out.append((f"lap2d16_{wt}", L16, None, 6, P.random_x0(L16.shape[0], 6, 15), 1, 0,
+                    ["-gcge_w_orth_type", wt], dict(w_orth_type=wt)))
+        out.append((f"lap3dmass8_{wt}", A8, B8, 10, P.random_x0(A8.shape[0], 10, 16), 1, 0,
+                    ["-gcge_w_orth_type", wt, "-gcge_ev_max_it", "60"], dict(w_orth_type=wt, ev_max_it=60)))
+    # scbgs with W in blocks of 2 columns (w_orth_block_size) and two MGS sweeps per block
+    out.append(("lap2d32_scbgs_blk", L32, None, 12, P.random_x0(L32.shape[0], 12, 17), 1, 0,
+                ["-gcge_w_orth_type", "scbgs", "-gcge_w_orth_block_size", "2",
+                 "-gcge_scbgs_wself_max_reorth_time", "2", "-gcge_ev_max_it", "80"],
+                dict(w_orth_type="scbgs", ev_max_it=80, w_orth_block_size=2, scbgs_wself_max_reorth_time=2)))
+    # soft locking: Rayleigh-Ritz pairs from the first unconverged one on
+    # (opt_rr_eig_partly, gcge_rayleighritz.c:380-394)
+    out.append(("lap2d32_rrpartly", L32, None, 20, P.random_x0(L32.shape[0], 20, 18), 1, 0,
+                ["-gcge_opt_rr_eig_partly", "1", "-gcge_ev_tol", "1e-10", "-gcge_ev_max_it", "80"],
+                dict(opt_rr_eig_partly=1, ev_tol=1e-10, ev_max_it=80)))
+    out.append(("lap3dmass8_rrpartly", A8, B8, 10, P.random_x0(A8.shape[0], 10, 19), 1, 0,
+                ["-gcge_opt_rr_eig_partly", "1", "-gcge_ev_max_it", "60"],
+                dict(opt_rr_eig_partly=1, ev_max_it=60)))
     # rank-deficient X0: duplicated columns exercise the deflation swaps
     X = P.random_x0(L16.shape[0], 6, 13)
     X[:, 3] = X[:, 1]
@@ -92,11 +112,13 @@ def cases():
     return out
 
 
-def main():
+def main(names=()):
     os.environ["MKL_NUM_THREADS"] = "1"
     if not O.ref_available():
         O.build_ref()
     for name, A, B, nev, X0, seed, dirich, flags, okw in cases():
+        if names and name not in names:
+            continue
         r = O.ref_solve(A, B, nev, X0=X0, flags=flags, seed=seed, dirichlet=dirich)
         its, notices = O.parse_trace(r["stdout"])
         hist_ev = np.array([it["eval"] for it in its]) if its else np.zeros((0, nev))
@@ -124,4 +146,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

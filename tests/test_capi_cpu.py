@@ -188,3 +188,127 @@ def test_dense_eigensolver_range_on_degenerate_clusters(cluster):
     np.testing.assert_allclose(ev, lam[il - 1:iu], rtol=1e-12, atol=1e-13)
     assert np.abs(V.T @ V - np.eye(m)).max() < 1e-12
     assert np.abs(A @ V - V * ev).max() < 1e-12 * np.abs(lam).max() * 10
+
+
+def _conv_setup(nev, res_values):
+    """GCGE_PARA (conv_type R, ev_tol 1e-8) and a workspace for
+    GCGE_CheckConvergence, with an ops table whose MultiVecResidualNorms
+    returns ||r|| = res_values[idx], ||x|| = 1 (no device involved)."""
+    L = capi.load()
+    p = C.POINTER(capi.GCGE_PARA)()
+    L.GCGE_PARA_Create(C.byref(p))
+    L.GCGE_PARA_SetNumEigen(p, nev)
+    L.GCGE_PARA_Setup(p)
+    ws = capi.GCGE_WORKSPACE()
+    unlock = (C.c_int * nev)(*range(nev))
+    ws.unlock = C.cast(unlock, capi.IP)
+    ws.max_dim_x = nev + 3
+    ops = capi.GCGE_OPS()
+    calls = []
+
+    def resnorms(A, B, X, idx, k, lam, rn, xn, bn, o):
+        for t in range(k):
+            calls.append(idx[t])
+            rn[t] = res_values[idx[t]]
+            xn[t] = 1.0
+            if bn:
+                bn[t] = 1.0
+
+    cb = capi.F_MultiVecResidualNorms(resnorms)
+    ops.MultiVecResidualNorms = C.cast(cb, C.c_void_p)
+    keep = (unlock, cb)
+    return L, p, ws, ops, calls, keep
+
+
+def test_nan_residual_is_not_converged():
+    """The reference tests residual > ev_tol (src/gcge_eigsol.c:630), so a
+    NaN residual passes as converged; here it stays unconverged, becomes the
+    largest residual and latches error 190."""
+    nev = 6
+    res = [1e-12, float("nan"), 1e-12, 1e-12, float("inf"), 1e-12]
+    L, p, ws, ops, calls, keep = _conv_setup(nev, res)
+    ev = np.arange(1.0, nev + 1.0) * 0.1
+    L.GCGE_HIP_ClearError()
+    L.GCGE_CheckConvergence(None, None, ev.ctypes.data_as(capi.DP), None, C.byref(ops), p, C.byref(ws))
+    P = p.contents
+    assert P.num_unlock == 2
+    assert [ws.unlock[i] for i in range(2)] == [1, 4]
+    assert np.isnan(P.max_res)
+    code, msg = capi.last_error()
+    assert code == 190 and "eigenpair 1" in msg
+    L.GCGE_HIP_ClearError()
+    L.GCGE_PARA_Free(C.byref(p))
+
+
+def test_finite_residuals_unchanged_rule():
+    """Finite residuals follow the reference's rule, including the early exit
+    eight pairs past the last converged one (src/gcge_eigsol.c:637-645)."""
+    nev = 14
+    res = [1e-12] * 2 + [1.0] * 12
+    L, p, ws, ops, calls, keep = _conv_setup(nev, res)
+    ev = np.arange(1.0, nev + 1.0) * 0.1
+    L.GCGE_HIP_ClearError()
+    L.GCGE_CheckConvergence(None, None, ev.ctypes.data_as(capi.DP), None, C.byref(ops), p, C.byref(ws))
+    P = p.contents
+    assert P.num_unlock == 12
+    assert capi.last_error()[0] == 0
+    # pairs 0-1 converge (last converged position 1); the check stops at the
+    # first unconverged position past 1 + 8, so pairs 11.. are never visited
+    assert sorted(set(calls)) == list(range(11))
+    assert P.num_iter == 1
+    L.GCGE_PARA_Free(C.byref(p))
+
+
+def test_w_orth_types_parse():
+    """-gcge_w_orth_type accepts the reference's names (multi, gs, bgs, cbgs,
+    scbgs) and this build's cholqr."""
+    L = capi.load()
+    for wt in (b"multi", b"gs", b"bgs", b"cbgs", b"scbgs", b"cholqr"):
+        p = C.POINTER(capi.GCGE_PARA)()
+        L.GCGE_PARA_Create(C.byref(p))
+        args = [b"prog", b"-gcge_w_orth_type", wt]
+        argv = (C.c_char_p * len(args))(*args)
+        assert L.GCGE_PARA_SetFromCommandLine(p, len(args), argv) == 0
+        assert p.contents.w_orth_type == wt
+        L.GCGE_PARA_Free(C.byref(p))
+
+
+def _read_text_csr(path):
+    """The reference's reader (CSR_ReadMatFile, app/csr/gcge_app_csr.c:185-236):
+    nrow, ncol, nnz, then rowptr, col, val, whitespace separated."""
+    toks = open(path).read().split()
+    n, m, nnz = int(toks[0]), int(toks[1]), int(toks[2])
+    ia = np.array(toks[3:4 + n], dtype=np.int64)
+    ja = np.array(toks[4 + n:4 + n + nnz], dtype=np.int64)
+    aa = np.array([float(t) for t in toks[4 + n + nnz:4 + n + 2 * nnz]])
+    return n, m, nnz, ia, ja, aa
+
+
+@pytest.mark.parametrize("binary", [0, 1])
+def test_write_mat_file_formats(tmp_path, binary):
+    """HIP_WriteMatFile (host only): the text form is the reference's reader
+    format with values that read back exactly; the binary form is the
+    "GCGECSR1" header and the three arrays."""
+    from gcge_hip import problems as P
+    from gcge_hip import write_matrix_file
+    A, B = P.lap3d_mass(5)
+    path = tmp_path / ("B.bin" if binary else "B.txt")
+    write_matrix_file(path, B, binary=bool(binary))
+    if not binary:
+        n, m, nnz, ia, ja, aa = _read_text_csr(path)
+    else:
+        raw = path.read_bytes()
+        assert raw[:8] == b"GCGECSR1"
+        n, m, nnz, zero = np.frombuffer(raw[8:24], dtype="<i4")
+        assert zero == 0
+        off = 24
+        ia = np.frombuffer(raw[off:off + 4 * (n + 1)], dtype="<i4")
+        off += 4 * (n + 1)
+        ja = np.frombuffer(raw[off:off + 4 * nnz], dtype="<i4")
+        off += 4 * nnz
+        aa = np.frombuffer(raw[off:off + 8 * nnz], dtype="<f8")
+        assert off + 8 * nnz == len(raw)
+    assert (n, m, nnz) == (B.shape[0], B.shape[1], B.nnz)
+    np.testing.assert_array_equal(ia, B.indptr)
+    np.testing.assert_array_equal(ja, B.indices)
+    np.testing.assert_array_equal(aa, B.data)   # bit for bit

@@ -135,3 +135,85 @@ def test_gradient_512x512_autograd_against_fd():
         fd = (Lp - Lm) / (2 * h)
         ad = float(np.dot(grad, v))
         assert abs(ad - fd) <= 1e-6 * abs(fd), (ad, fd)
+
+
+def laplace_plus_mass(m, kappa, rho):
+    """A(kappa) = the FV diffusion operator (x-faces x1.3) and B(rho) = I +
+    0.1 L(rho), L(rho) the FV operator with coefficients rho: both depend on
+    cell parameters, B through off-diagonal entries too.  Returns A, B and
+    the Jacobians of their stored values with respect to theta = [kappa; rho]
+    (2 m^2 parameters): A's rows for kappa, B's for rho."""
+    n = m * m
+    A, _ = P.vardiff2d(m, theta=kappa, anis=ANIS)
+    L = P.fv_diffusion(rho)
+    B = L.copy()
+    B.data = 0.1 * B.data
+    diag = np.repeat(np.arange(n), np.diff(B.indptr)) == B.indices
+    B.data[diag] += 1.0
+    ja = P.fv_diffusion_jacobian(kappa, A, face_scale=(1.0, ANIS))
+    jl = P.fv_diffusion_jacobian(rho, L)
+    # stack to 2n parameter rows: A depends on the first n, B on the last n
+    ja2 = (np.concatenate([ja[0], np.full(n, ja[0][-1], dtype=np.int32)]), ja[1], ja[2])
+    jb2 = (np.concatenate([np.zeros(n, dtype=np.int32), jl[0]]), jl[1], 0.1 * jl[2])
+    return A, B, ja2, jb2
+
+
+def test_generalized_gradient_dense_and_fd():
+    """d(sum of the lowest 16 eigenvalues of A x = lambda B x)/d theta from
+    GCGE_HIP_EigGradientGeneralized, x_i^T (dA/dtheta - lambda_i dB/dtheta)
+    x_i with B-normalized eigenvectors from the device solver, against the
+    same formula with eigenvectors from dense LAPACK (scipy.linalg.eigh(A,
+    B)) and against FP64 central differences of the dense loss, for
+    parameters of A and of B; 1e-6 relative; non-degenerate spectrum
+    enforced."""
+    from gcge_hip import eig_gradient_generalized
+    m = 32
+    n = m * m
+    kappa = P.vardiff2d_theta(m, var=0.25, seed=31)
+    rho = P.vardiff2d_theta(m, var=0.25, seed=32)
+    A, B, ja, jb = laplace_plus_mass(m, kappa, rho)
+    w, V = sla.eigh(A.toarray(), B.toarray(), subset_by_index=[0, NEV])
+    assert_simple(w)
+    r = solve(A, B, NEV, X0=P.random_x0(n, NEV, 33), conv_type="NS", ev_tol=1e-11, ev_max_it=2000)
+    assert r["num_unlock"] == 0
+    np.testing.assert_allclose(r["eval"], w[:NEV], rtol=1e-10)
+    X = r["evec"]
+    np.testing.assert_allclose(X.T @ (B @ X), np.eye(NEV), atol=1e-10)
+    mA, mB = Matrix(A), Matrix(B)
+    g_dev = eig_gradient_generalized(mA, mB, X, r["eval"], np.ones(NEV), 2 * n, ja, jb)
+    mA.close()
+    mB.close()
+
+    def sampled(M, Vv, wts):
+        rows = np.repeat(np.arange(n), np.diff(M.indptr))
+        ge = np.zeros(M.nnz)
+        for i in range(NEV):
+            ge += wts[i] * Vv[rows, i] * Vv[M.indices, i]
+        return ge
+
+    def gather(j, ge):
+        jrp, jnz, jval = j
+        return np.array([np.dot(jval[jrp[c]:jrp[c + 1]], ge[jnz[jrp[c]:jrp[c + 1]]]) for c in range(2 * n)])
+
+    g_dense = gather(ja, sampled(A, V, np.ones(NEV))) - gather(jb, sampled(B, V, w[:NEV]))
+    scale = np.abs(g_dense).max()
+    np.testing.assert_allclose(g_dev, g_dense, rtol=1e-6, atol=1e-9 * scale)
+    assert np.abs(g_dense[n:]).max() > 1e-3 * scale  # the B term is exercised
+
+    def loss(k2, r2):
+        A2, B2, _, _ = laplace_plus_mass(m, k2, r2)
+        return float(np.sum(sla.eigh(A2.toarray(), B2.toarray(), eigvals_only=True, subset_by_index=[0, NEV - 1])))
+
+    rng = np.random.default_rng(34)
+    h = 1e-6
+    for _ in range(3):
+        v = rng.uniform(0.0, 1.0, 2 * n)
+        vk, vr = v[:n].reshape(m, m), v[n:].reshape(m, m)
+        fd = (loss(kappa + h * vk, rho + h * vr) - loss(kappa - h * vk, rho - h * vr)) / (2 * h)
+        ad = float(np.dot(g_dev, v))
+        assert abs(ad - fd) <= 1e-6 * abs(fd), (ad, fd)
+    # B's parameters alone (the -lambda dB/dtheta term)
+    v = rng.uniform(0.0, 1.0, n).reshape(m, m)
+    fd = (loss(kappa, rho + h * v) - loss(kappa, rho - h * v)) / (2 * h)
+    ad = float(np.dot(g_dev[n:], v.ravel()))
+    assert abs(ad - fd) <= 1e-6 * abs(fd), (ad, fd)

@@ -126,6 +126,51 @@ def test_spmm_row_gather_fp64_bit_exact(ops, monkeypatch, name, k):
     np.testing.assert_allclose(d, np.einsum("ij,ij->j", X, Y), rtol=RTOL_RED)
 
 
+@pytest.mark.parametrize("name", list(MATS))
+@pytest.mark.parametrize("kc", ["", "1", "2", "4", "8"])
+def test_spmm_halo_fp64_bit_exact(ops, monkeypatch, name, kc):
+    """Staged row blocks (k_spmm_halo): every X row a 256-row block reads is
+    copied to LDS once per column pass; sums in CSR order from 0, so the same
+    bits as the oracle's CSR_MatDotVec loop.  mass27 exercises rows longer
+    than the register-held entries, chunglu the long-row kernel beside it;
+    kc forces the columns per LDS pass."""
+    A = MATS[name]
+    n = A.shape[0]
+    if kc:
+        monkeypatch.setenv("GCGE_HIP_HALO_KC", kc)
+    mA = _matrix_with_gather(A, "halo", monkeypatch)
+    for k in (1, 3, 9, 70, 130):
+        X = rng(k + 21).standard_normal((n, k))
+        x = MultiVec(mA, k, data=X)
+        y, y2 = MultiVec(mA, k), MultiVec(mA, k)
+        ops.matdot(mA, x, y)
+        Y = O.csr_matmultivec(A, X)
+        np.testing.assert_array_equal(y.get(), Y)
+        d = ops.matdot_dot(mA, x, y2)
+        np.testing.assert_array_equal(y2.get(), Y)
+        np.testing.assert_allclose(d, np.einsum("ij,ij->j", X, Y), rtol=RTOL_RED)
+        # FP32 storage: identical to the direct-gather kernel's
+        xf = MultiVec(mA, k, F32, data=X)
+        yf = MultiVec(mA, k, F32)
+        ops.matdot(mA, xf, yf)
+        mC = _matrix_with_gather(A, "columns", monkeypatch)
+        xc = MultiVec(mC, k, F32, data=X)
+        yc = MultiVec(mC, k, F32)
+        ops.matdot(mC, xc, yc)
+        np.testing.assert_array_equal(yf.get(), yc.get())
+        for m in (x, y, y2, xf, yf, xc, yc):
+            m.close()
+        mC.close()
+    # column offsets
+    X = rng(14).standard_normal((n, 9))
+    x, y = MultiVec(mA, 9, data=X), MultiVec(mA, 12)
+    ops.matdot(mA, x, y, xs=2, ys=5, k=6)
+    Y = y.get()
+    np.testing.assert_array_equal(Y[:, 5:11], O.csr_matmultivec(A, X[:, 2:8]))
+    assert np.all(Y[:, :5] == 0) and np.all(Y[:, 11:] == 0)
+    mA.close()
+
+
 @pytest.mark.parametrize("name", ["vardiff2d", "chunglu"])
 @pytest.mark.parametrize("k", [2, 6, 40, 100])
 def test_spmm_row_gather_fp32_matches_column_gather(ops, monkeypatch, name, k):

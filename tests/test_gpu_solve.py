@@ -59,17 +59,56 @@ def ns_residual(A, B, X, ev):
 # outcomes: the duplicate kept (lap2d16_dupx0) or deflated, which is the
 # reference's deterministic zero-column path (lap2d16_zerox0).
 ROUNDING_SENSITIVE = {"lap2d16_dupx0": "lap2d16_zerox0"}
+# After a deflated initial column the reference continues with dim_x < nev
+# (gcge_eigsol.c:351): its run (lap2d16_zerox0) ends with a negative
+# "eigenvalue" of an SPD matrix and a NaN residual for the last pair, which
+# its residual > ev_tol test counts as converged.  The device path leaves a
+# NaN residual unconverged and stops with error 190.
+NAN_IN_REFERENCE = {"lap2d16_zerox0"}
+
+
+# The reference's classical block orthonormalization (w_orth_type "cbgs",
+# src/gcge_orthogonal.c:755-963) orthonormalizes W by W Y Theta^(-1/2) from
+# the eigenpairs of W^T B W without a projection afterwards; once W is small
+# (residuals near 1e-3) the scaling amplifies its rounding-level components
+# along X and P, and the basis loses B-orthogonality.  In the reference's
+# own runs (fixtures) the Ritz values then fall below the spectrum and the
+# solve ends unconverged.  What follows the loss is decided by rounding, so
+# the device run is compared on the iterations before it, and must not
+# report convergence where the reference did not.
+BREAKDOWN = {"lap2d16_cbgs": 6, "lap3dmass8_cbgs": 5}
+
+
+def _trace(capfd):
+    its, _ = O.parse_trace(capfd.readouterr().out)
+    return its
 
 
 @pytest.mark.parametrize("path", GOLD, ids=[os.path.basename(p)[:-4] for p in GOLD])
-def test_fp64_matches_reference_fixture(path):
+def test_fp64_matches_reference_fixture(path, capfd):
     d, A, B, X0, kw = load_case(path)
     name = os.path.basename(path)[:-4]
+    if name in BREAKDOWN:
+        k = BREAKDOWN[name]
+        capfd.readouterr()
+        r = solve(A, B, int(d["nev"]), X0=X0, seed=int(d["seed"]), print_eval=1, **kw)
+        its = _trace(capfd)
+        he = np.array([it["eval"] for it in its])
+        np.testing.assert_allclose(he[:k], d["hist_eval"][:k], rtol=1e-8)
+        assert r["num_unlock"] > 0 and int(d["num_unlock"]) > 0
+        return
+    if name in NAN_IN_REFERENCE:
+        assert np.isnan(d["res"]).any()
+        with pytest.raises(RuntimeError, match="error 190"):
+            solve(A, B, int(d["nev"]), X0=X0, seed=int(d["seed"]), **kw)
+        return
     if name in ROUNDING_SENSITIVE:
-        alt = np.load(os.path.join(os.path.dirname(path), ROUNDING_SENSITIVE[name] + ".npz"))
-        r = solve(A, B, int(d["nev"]), X0=X0, seed=int(d["seed"]), **kw)
-        ok = [np.allclose(r["eval"], ref, rtol=1e-6, atol=0) for ref in (d["eval"], alt["eval"])]
-        assert any(ok), (r["eval"], d["eval"], alt["eval"])
+        try:
+            r = solve(A, B, int(d["nev"]), X0=X0, seed=int(d["seed"]), **kw)
+        except RuntimeError as e:  # the deflated outcome (see NAN_IN_REFERENCE)
+            assert "error 190" in str(e)
+            return
+        assert np.allclose(r["eval"], d["eval"], rtol=1e-6, atol=0), (r["eval"], d["eval"])
         return
     nev = int(d["nev"])
     r = solve(A, B, nev, X0=X0, seed=int(d["seed"]), **kw)
@@ -357,3 +396,202 @@ def test_full_rayleigh_ritz_decomposition_matches_reference(name):
     assert np.all(rel_residual_R(A, B, r["evec"], r["eval"]) <= kw.get("ev_tol", 1e-8) * 1.5)
     assert abs(r["num_iter"] - int(d["num_iter"])) <= 5, (r["num_iter"], int(d["num_iter"]))
 
+
+
+def test_orth_variant_iteration_histories_match_reference(capfd):
+    """bgs / scbgs (src/gcge_orthogonal.c:352-528, 967-1279) and soft locking
+    (opt_rr_eig_partly, src/gcge_rayleighritz.c:380-394): the per-iteration
+    Ritz values of the device run follow the reference's to 1e-8 (early
+    iterates carry the summation-order differences) until both converge."""
+    for name in ("lap2d16_bgs", "lap3dmass8_scbgs", "lap2d32_scbgs_blk", "lap2d32_rrpartly", "lap3dmass8_rrpartly"):
+        d, A, B, X0, kw = load_case(os.path.join(os.path.dirname(__file__), "golden", name + ".npz"))
+        capfd.readouterr()
+        r = solve(A, B, int(d["nev"]), X0=X0, seed=int(d["seed"]), print_eval=1, **kw)
+        he = np.array([it["eval"] for it in _trace(capfd)])
+        k = min(len(he), len(d["hist_eval"]))
+        assert k >= len(d["hist_eval"]) - 2, (name, len(he), len(d["hist_eval"]))
+        np.testing.assert_allclose(he[:k], d["hist_eval"][:k], rtol=1e-8, err_msg=name)
+        assert r["num_unlock"] == 0
+
+
+def test_nan_in_matrix_latches_error_instead_of_converging():
+    """A NaN value in A makes the residuals NaN: the pairs stay unconverged,
+    a non-finite error (190-192) is latched and the solve stops (the reference's residual >
+    ev_tol test would count them as converged, src/gcge_eigsol.c:630)."""
+    from gcge_hip import capi
+    A, _ = P.lap2d(16)
+    A = A.copy()
+    A.data[5] = np.nan
+    X0 = P.random_x0(A.shape[0], 6, 1)
+    with pytest.raises(RuntimeError, match=r"error 19[0-2]"):
+        solve(A, None, 6, X0=X0, ev_max_it=50)
+    assert capi.last_error()[0] == 0  # raise_on_error cleared it
+
+
+def test_mixed_nan_matrix_returns_error():
+    A, _ = P.lap2d(16)
+    A = A.copy()
+    A.data[5] = np.nan
+    X0 = P.random_x0(A.shape[0], 6, 1)
+    with pytest.raises(RuntimeError, match=r"error 19[0-2]"):
+        solve(A, None, 6, X0=X0, precision="mixed", ev_max_it=50, conv_type="NS")
+
+
+def test_mixed_reports_unconverged_pairs_on_budget():
+    """With a refinement budget too small, the mixed solve returns
+    GCGE_HIP_NOT_CONVERGED and num_unlock counts the pairs above the
+    tolerance at its final check of every pair."""
+    from gcge_hip import capi
+    A, _ = P.vardiff3d(16, seed=1)
+    nev = 12
+    X0 = P.random_x0(A.shape[0], nev, 4)
+    r = solve(A, None, nev, X0=X0, precision="mixed", ev_max_it=200, conv_type="NS", ev_tol=1e-10,
+              mixed=dict(fp32_max_it=2, fp64_max_it=2))
+    assert r["rc"] == capi.GCGE_HIP_NOT_CONVERGED
+    ns = ns_residual(A, None, r["evec"], r["eval"])
+    assert r["num_unlock"] == int(np.sum(ns > 1e-10 * (1 + 1e-6))) > 0
+    np.testing.assert_allclose(r["res"], ns, rtol=1e-6)
+    assert r["info"]["refine_passes"] == 1
+
+
+def test_mixed_skips_fp32_phase_out_of_range():
+    """A matrix scaled below FP32's safe range (Gershgorin bound 8e-20 <
+    1e-18: squares of its products underflow in FP32) runs the FP64
+    iteration alone, without FP32 inner solves, and still converges.  (A
+    large scale, 1e20, is no test of this: GCGE's W = A^-1 x / ... has norm
+    1/lambda there, below the absolute orth_zero_tol, in the reference as
+    here.)"""
+    A, _ = P.lap2d(16)
+    A = (A * 1e-20).tocsr()
+    X0 = P.random_x0(A.shape[0], 6, 2)
+    r = solve(A, None, 6, X0=X0, precision="mixed", ev_max_it=200, conv_type="NS", ev_tol=1e-9)
+    assert r["info"]["fp32_failed"] == 2 and r["info"]["fp32_iters"] == 0
+    assert r["num_unlock"] == 0 and r["rc"] == 0
+    np.testing.assert_allclose(r["eval"], 1e-20 * P.lap2d_eigs(16, 6), rtol=1e-10)
+
+
+def test_set_values_refreshes_jacobi_diagonal():
+    """HIP_MatSetValues refreshes the FP32 copy and the Jacobi diagonal of the
+    preconditioned inner solves: a solve after set_values(2 val) equals a
+    solve on a matrix created with 2 val (same eigenvalues, iterations)."""
+    from gcge_hip import Matrix
+    A, _ = P.vardiff2d(32, seed=2)
+    nev = 8
+    X0 = P.random_x0(A.shape[0], nev, 3)
+    kw = dict(conv_type="NS", ev_tol=1e-9, ev_max_it=200, bcg_precond=1, w_orth_type="cholqr")
+    M = Matrix(A)
+    r1 = solve(M, None, nev, X0=X0, **kw)
+    M.set_values(2.0 * A.data)
+    r2 = solve(M, None, nev, X0=X0, **kw)
+    M2 = Matrix((2.0 * A).tocsr())
+    r3 = solve(M2, None, nev, X0=X0, **kw)
+    assert r2["num_unlock"] == 0 and r3["num_unlock"] == 0
+    np.testing.assert_allclose(r2["eval"], r3["eval"], rtol=1e-12)
+    np.testing.assert_allclose(r2["eval"], 2.0 * r1["eval"], rtol=1e-10)
+    assert r2["num_iter"] == r3["num_iter"]
+    # and the mixed path, whose FP32 phase reads the FP32 copy
+    r4 = solve(M, None, nev, X0=X0, precision="mixed", conv_type="NS", ev_tol=1e-9, ev_max_it=200)
+    r5 = solve(M2, None, nev, X0=X0, precision="mixed", conv_type="NS", ev_tol=1e-9, ev_max_it=200)
+    np.testing.assert_allclose(r4["eval"], r5["eval"], rtol=1e-12)
+    assert r4["num_iter"] == r5["num_iter"]
+    M.close()
+    M2.close()
+
+
+def test_solver_reuse_gives_identical_results():
+    """A set-up solver solves again from a new initial block with the same
+    result as a fresh solver (workspace reused, per-solve state restarted),
+    also for the mixed path, whose FP32-phase solver is kept."""
+    from gcge_hip import Solver
+    A, _ = P.vardiff2d(32, seed=5)
+    nev = 8
+    Xa = P.random_x0(A.shape[0], nev, 6)
+    Xb = P.random_x0(A.shape[0], nev, 7)
+    for prec in ("fp64", "mixed"):
+        kw = dict(conv_type="NS", ev_tol=1e-9, ev_max_it=200)
+        s = Solver(A, None, nev, precision=prec, **kw)
+        ra, rb, ra2 = s.solve(Xa), s.solve(Xb), s.solve(Xa)
+        s.close()
+        fa = solve(A, None, nev, X0=Xa, precision=prec, **kw)
+        fb = solve(A, None, nev, X0=Xb, precision=prec, **kw)
+        for x, y in ((ra, fa), (rb, fb), (ra2, fa)):
+            assert x["num_iter"] == y["num_iter"] and x["num_unlock"] == 0
+            np.testing.assert_array_equal(x["eval"], y["eval"])
+
+
+def test_mixed_power_law_graph_matches_reference():
+    """The mixed path (FP32 phase + FP64 refinement) on the 200,000-vertex
+    Chung-Lu graph: eigenvalues within 1e-10 of the reference's FP64 run and
+    every pair at ||Ax - lambda x|| / (|lambda| ||x||) <= 1e-8."""
+    if not os.path.exists(LARGE_CL):
+        pytest.skip("fixture not generated (make_chunglu200k.py)")
+    d = np.load(LARGE_CL)
+    n, nev = int(d["n"]), int(d["nev"])
+    A, _ = P.chung_lu(n, seed=int(d["graph_seed"]))
+    X0 = P.random_x0(n, nev, int(d["x0_seed"]))
+    r = solve(A, None, nev, X0=X0, precision="mixed", ev_max_it=250, ev_tol=1e-8, conv_type="NS")
+    assert r["num_unlock"] == 0 and r["rc"] == 0
+    np.testing.assert_allclose(r["eval"], d["eval"], rtol=EV_RTOL)
+    assert ns_residual(A, None, r["evec"], r["eval"]).max() <= 1e-8 * (1 + 1e-6)
+
+
+FULLSIZE = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "large", "*2d*_ref.npz")))
+
+
+@pytest.mark.parametrize("path", FULLSIZE, ids=[os.path.basename(p)[:-8] for p in FULLSIZE])
+def test_fp64_full_size_workloads_match_reference(path):
+    """The 128 x 128 Laplacian (n = 16,384, nev 20) and the 512 x 512
+    variable-coefficient diffusion (n = 262,144, nev 16) at full size, from
+    the reference's own runs (tests/golden/large/make_fullsize.py): the FP64
+    drop-in takes the same outer iterations (within 2), reaches the
+    reference's eigenvalues within 1e-10 relative, and every pair meets the
+    reference's "R" tolerance, chosen so that it implies
+    ||Ax - lambda x|| / (|lambda| ||x||) <= 1e-8."""
+    d = np.load(path)
+    m, nev = int(d["m"]), int(d["nev"])
+    A = P.lap2d(m)[0] if str(d["gen"]) == "lap2d" else P.vardiff2d(m, seed=int(d["gen_seed"]))[0]
+    ints = np.array([A.shape[0], A.nnz, int(A.indptr.astype(np.int64).sum()),
+                     int(A.indices.astype(np.int64).sum())], dtype=np.int64)
+    assert np.array_equal(ints, d["fp_ints"]) and np.isclose(A.data.sum(), float(d["fp_data_sum"]), rtol=1e-12)
+    X0 = P.random_x0(A.shape[0], nev, int(d["x0_seed"]))
+    tol = float(d["ev_tol"])
+    r = solve(A, None, nev, X0=X0, precision="fp64", ev_max_it=2000, ev_tol=tol, conv_type="R")
+    assert r["num_unlock"] == 0 and int(d["num_unlock"]) == 0
+    assert abs(r["num_iter"] - int(d["num_iter"])) <= 2, (r["num_iter"], int(d["num_iter"]))
+    np.testing.assert_allclose(r["eval"], d["eval"], rtol=EV_RTOL)
+    assert rel_residual_R(A, None, r["evec"], r["eval"]).max() <= tol * (1 + 1e-6)
+    assert ns_residual(A, None, r["evec"], r["eval"]).max() <= 1e-8
+    # and the mixed path reaches the same eigenvalues at the NS criterion
+    rm = solve(A, None, nev, X0=X0, precision="mixed", ev_max_it=2000, ev_tol=1e-8, conv_type="NS")
+    assert rm["num_unlock"] == 0 and rm["rc"] == 0
+    np.testing.assert_allclose(rm["eval"], d["eval"], rtol=EV_RTOL)
+    assert ns_residual(A, None, rm["evec"], rm["eval"]).max() <= 1e-8 * (1 + 1e-6)
+
+
+@pytest.mark.parametrize("binary", [False, True])
+def test_matrix_files_round_trip_and_solve(tmp_path, binary):
+    """HIP_ReadMatFile on the reference's test matrices A_5 / M_5 (the arrays
+    of the A5M5_example fixture, written back out in the reference's text
+    format, or in the binary form): the device matrix equals the original
+    (SpMM bit for bit) and the reference's example solve reaches its
+    eigenvalues (example/csr_solver.c:41-61: nev 10, Dirichlet rows)."""
+    from gcge_hip import write_matrix_file
+    path = os.path.join(os.path.dirname(__file__), "golden", "A5M5_example.npz")
+    d, A, B, X0, kw = load_case(path)
+    fa, fb = tmp_path / "A_5", tmp_path / "M_5"
+    write_matrix_file(fa, A, binary=binary)
+    write_matrix_file(fb, B, binary=binary)
+    mA, mB = Matrix.from_file(fa), Matrix.from_file(fb)
+    assert (mA.n, mA.nnz, mB.nnz) == (A.shape[0], A.nnz, B.nnz)
+    ops = Ops()
+    X = np.random.default_rng(1).standard_normal((A.shape[0], 3))
+    for M, mM in ((A, mA), (B, mB)):
+        x, y = MultiVec(mM, 3, data=X), MultiVec(mM, 3)
+        ops.matdot(mM, x, y)
+        np.testing.assert_array_equal(y.get(), O.csr_matmultivec(M, X))
+    ops.close()
+    r = solve(mA, mB, int(d["nev"]), seed=int(d["seed"]), **kw)
+    np.testing.assert_allclose(r["eval"], d["eval"], rtol=EV_RTOL)
+    assert r["num_unlock"] == 0
+    mA.close()
+    mB.close()
