@@ -70,6 +70,7 @@ bool ensure_ctx()
     hipDeviceProp_t prop;
     if (check(hipGetDeviceProperties(&prop, g_ctx.device), "hipGetDeviceProperties"))
         g_ctx.num_cu = prop.multiProcessorCount;
+    if (!noalloc_setup(g_ctx.device)) return false;
     g_ctx.init = true;
     return true;
 }

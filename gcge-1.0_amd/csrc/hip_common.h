@@ -106,6 +106,9 @@ struct Ctx {
 
 Ctx &ctx();
 bool ensure_ctx();
+// device address of this GPU's no-allocation scratch array (noalloc.hip),
+// looked up once per device at context set-up
+bool noalloc_setup(int device);
 void *scratch(size_t bytes);           // device scratch of at least bytes
 void *row_stage(size_t bytes);         // device staging of row-major X copies (SpMM), at least bytes
 double *host_buf(size_t count);        // pinned host doubles

@@ -186,6 +186,7 @@ SIGNATURES = {
     "GCGE_HIP_EigGradient": (c_int, [c_void_p, c_void_p, c_int, DP, c_int, IP, IP, DP, DP]),
     "GCGE_HIP_EigGradientGeneralized": (c_int, [c_void_p, c_void_p, c_void_p, c_int, DP, DP, c_int, IP, IP, DP, IP,
                                                 IP, DP, DP]),
+    "GCGE_HIP_MultiVecInnerProdDiagNoAlloc": (c_int, [c_void_p, c_void_p, DP, IP, IP]),
     "GCGE_HIP_GetLastError": (c_int, [c_char_p, c_int]),
     "GCGE_HIP_ClearError": (None, []),
     "GCGE_HIP_Synchronize": (c_int, []),

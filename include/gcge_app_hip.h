@@ -173,6 +173,20 @@ GCGE_INT GCGE_HIP_EigGradientGeneralized(HIP_MAT *A, HIP_MAT *B, void **X, GCGE_
                                          const GCGE_INT *jb_rowptr, const GCGE_INT *jb_nz,
                                          const GCGE_DOUBLE *jb_val, GCGE_DOUBLE *grad);
 
+/* ---- no allocation (no counterpart in the reference) ---- */
+
+/* For callers that cannot allocate device or pinned memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * MultiVecInnerProdDiag slot, out[j] = <X_{start[0]+j}, Y_{start[1]+j}>
+ * (j < end[0] - start[0]), FP64 accumulation, results in host memory out,
+ * with the same kernels and summation order (the same bits) and no
+ * allocation.  The partial sums use a fixed device array of the library
+ * (263 KiB per GPU, held until the process ends), so at most 128 columns
+ * per call (larger calls are rejected with error 202); calls that overlap
+ * on one GPU race on that array.  Returns 0 or the latched error code. */
+GCGE_INT GCGE_HIP_MultiVecInnerProdDiagNoAlloc(void **x, void **y, GCGE_DOUBLE *out, GCGE_INT *start,
+                                               GCGE_INT *end);
+
 /* ---- errors, synchronisation, counters ---- */
 
 /* Returns the latched error code (0 = none) and copies its message. */
