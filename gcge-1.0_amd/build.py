@@ -26,7 +26,7 @@ LIB = os.path.join(PKG, "lib", "libgcge_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["hip_kernels.hip", "hip_tiles.hip", "hip_backend.cpp", "gcge_core.cpp", "gcge_para.cpp", "gcge_dense.cpp",
-           "gcge_mixed.cpp", "noalloc.hip"]
+           "gcge_mixed.cpp", "noalloc.hip", "hip_order.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=gfx950", "-ffp-contract=off",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

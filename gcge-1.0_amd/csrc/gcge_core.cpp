@@ -743,6 +743,107 @@ void stable_multi_orth_insub(double *Vd, int ldV, int nrows, int start, int *end
     multi_orth_insub(Vd, ldV, nrows, start, end, op, ws, ops);
 }
 
+// ---- the other in-subspace orthonormalizations of ComputeP's coefficients
+// (p_orth_type "gs", "bgs", "scbgs"; gcge_xpw.c:363-385), host FP64 through
+// the dense slots, as the reference issues them.
+
+// GCGE_OrthonormalizationInSubspace, gcge_orthogonal.c:581-645: modified
+// Gram-Schmidt with reorthogonalization; zero columns replaced by the last.
+void gs_insub(double *Vd, int ldV, int nrows, int start, int *end, GCGE_ORTH_PARA *op, GCGE_OPS *ops)
+{
+    for (int current = start; current < *end; ++current) {
+        double *vc = Vd + (size_t)current * ldV;
+        double vout = ops->ArrayNorm(vc, nrows);
+        if (current != 0) {
+            int reorth_count = 0;
+            double vin, ratio;
+            do {
+                vin = vout;
+                for (int idx = 0; idx < current; idx++) {
+                    const double ip = ops->ArrayDotArray(Vd + (size_t)idx * ldV, vc, nrows);
+                    ops->ArrayAXPBY(-ip, Vd + (size_t)idx * ldV, 1.0, vc, nrows);
+                }
+                vout = ops->ArrayNorm(vc, nrows);
+                ratio = vout / vin;
+                reorth_count++;
+            } while (ratio < op->reorth_tol && reorth_count < op->max_reorth_time && vout > op->orth_zero_tol);
+        }
+        if (vout > op->orth_zero_tol) {
+            ops->ArrayScale(1.0 / vout, vc, nrows);
+        } else {
+            if (current != *end - 1) ops->ArrayCopy(Vd + (size_t)(*end - 1) * ldV, vc, nrows);
+            (*end)--;
+            current--;
+        }
+    }
+}
+
+// the self-orthonormalization sweep of the two block variants: each column
+// normalized, its components removed from the later ones (skipping
+// components below 10 orth_zero_tol), zero columns replaced by the last
+// (gcge_orthogonal.c:1538-1575, :1674-1714)
+void self_insub(double *Vd, int ldV, int nrows, int start, int *end, GCGE_ORTH_PARA *op, double *d, GCGE_OPS *ops)
+{
+    const double tol = op->orth_zero_tol;
+    for (int current = start; current < *end; ++current) {
+        int len = *end - current, one = 1, nr = nrows, ld = ldV;
+        double alpha = 1.0, beta = 0.0;
+        double *vc = Vd + (size_t)current * ldV;
+        ops->DenseMatDotDenseMat((char *)"T", (char *)"N", &len, &one, &nr, &alpha, vc, &ld, vc, &ld, &beta, d, &len);
+        const double norm = std::sqrt(d[0]);
+        if (norm > tol) {
+            ops->ArrayScale(1.0 / norm, vc, nrows);
+            for (int j = current + 1; j < *end; j++)
+                if (std::fabs(d[j - current] / norm) > 10.0 * tol)
+                    ops->ArrayAXPBY(-d[j - current] / norm, vc, 1.0, Vd + (size_t)j * ldV, nrows);
+        } else {
+            if (current != *end - 1) ops->ArrayCopy(Vd + (size_t)(*end - 1) * ldV, vc, nrows);
+            (*end)--;
+            current--;
+        }
+    }
+}
+
+// GCGE_BOrthonormalizationInSubspace, gcge_orthogonal.c:1501-1578: two
+// passes of column-by-column projection against V[0:start] (components
+// below 10 orth_zero_tol skipped) followed by the self sweep.
+void bgs_insub(double *Vd, int ldV, int nrows, int start, int *end, GCGE_ORTH_PARA *op, double *d, GCGE_OPS *ops)
+{
+    const double tol = op->orth_zero_tol;
+    for (int pass = 0; pass < 2; pass++) {
+        int len = *end - start, one = 1, nr = nrows, ld = ldV;
+        for (int current = 0; current < start; ++current) {
+            double alpha = 1.0, beta = 0.0;
+            ops->DenseMatDotDenseMat((char *)"T", (char *)"N", &len, &one, &nr, &alpha, Vd + (size_t)start * ldV, &ld,
+                                     Vd + (size_t)current * ldV, &ld, &beta, d, &len);
+            for (int j = start; j < *end; j++)
+                if (std::fabs(d[j - start]) > 10.0 * tol)
+                    ops->ArrayAXPBY(-d[j - start], Vd + (size_t)current * ldV, 1.0, Vd + (size_t)j * ldV, nrows);
+        }
+        self_insub(Vd, ldV, nrows, start, end, op, d, ops);
+    }
+}
+
+// GCGE_SCBOrthonormalizationInSubspace, gcge_orthogonal.c:1619-1718:
+// max_reorth_time block projections V2 -= V1 (V1^T V2) (the update covers ldV
+// rows, as in the reference), then one self sweep.
+void scbgs_insub(double *Vd, int ldV, int nrows, int start, int *end, GCGE_ORTH_PARA *op, double *d, GCGE_OPS *ops)
+{
+    if (start != 0) {
+        for (int rc = 0; rc < op->max_reorth_time; rc++) {
+            int len = *end - start, st = start, nr = nrows, ld = ldV;
+            double alpha = 1.0, beta = 0.0;
+            ops->DenseMatDotDenseMat((char *)"T", (char *)"N", &st, &len, &nr, &alpha, Vd, &ld,
+                                     Vd + (size_t)start * ldV, &ld, &beta, d, &st);
+            alpha = -1.0;
+            beta = 1.0;
+            ops->DenseMatDotDenseMat((char *)"N", (char *)"N", &ld, &len, &st, &alpha, Vd, &ld, d, &st, &beta,
+                                     Vd + (size_t)start * ldV, &ld);
+        }
+    }
+    self_insub(Vd, ldV, nrows, start, end, op, d, ops);
+}
+
 // ------------------------------------------------------------------------
 // CG solvers (src/gcge_cg.c)
 // ------------------------------------------------------------------------
@@ -1025,8 +1126,16 @@ void compute_p(double *coef, void **V, GCGE_OPS *ops, GCGE_PARA *para, GCGE_WORK
     const int orth_length = ldc - orth_start;
     p_start -= orth_start;
     int p_end = p_start + p_ncols;
-    stable_multi_orth_insub(coef + (size_t)orth_start * ldc + orth_start, ldc, orth_length, p_start, &p_end,
-                            para->orth_para, ws, ops);
+    double *sub = coef + (size_t)orth_start * ldc + orth_start;
+    const char *pt = para->p_orth_type;
+    if (std::strcmp(pt, "scbgs") == 0)
+        scbgs_insub(sub, ldc, orth_length, p_start, &p_end, para->orth_para, ws->subspace_dtmp, ops);
+    else if (std::strcmp(pt, "bgs") == 0)
+        bgs_insub(sub, ldc, orth_length, p_start, &p_end, para->orth_para, ws->subspace_dtmp, ops);
+    else if (std::strcmp(pt, "multi") == 0)
+        stable_multi_orth_insub(sub, ldc, orth_length, p_start, &p_end, para->orth_para, ws, ops);
+    else  // "gs", and any other name, as in the reference (gcge_xpw.c:381-385)
+        gs_insub(sub, ldc, orth_length, p_start, &p_end, para->orth_para, ops);
     p_end += orth_start;
     const int p_end_in_V_tmp = p_end - nev;
     {

@@ -708,6 +708,71 @@ extern "C" GCGE_INT HIP_WriteMatFile(const char *filename, GCGE_INT nrows, GCGE_
     return 0;
 }
 
+extern "C" HIP_MAT *HIP_MatCreateFromCSROrdered(GCGE_INT nrows, GCGE_INT ncols, GCGE_INT nnz, const GCGE_INT *rowptr,
+                                                const GCGE_INT *col, const GCGE_DOUBLE *val, const GCGE_INT *perm)
+{
+    if (nrows <= 0 || ncols != nrows || nnz < 0 || !rowptr || (nnz > 0 && (!col || !val)) ||
+        rowptr[nrows] != nnz) {
+        latch(110, "HIP_MatCreateFromCSROrdered: bad arguments (n=%d, m=%d, nnz=%d)", nrows, ncols, nnz);
+        return nullptr;
+    }
+    for (int i = 0; i < nrows; i++)
+        if (rowptr[i + 1] < rowptr[i] || rowptr[i] < 0) {
+            latch(112, "HIP_MatCreateFromCSROrdered: rowptr decreases at row %d", i);
+            return nullptr;
+        }
+    for (int e = 0; e < nnz; e++)
+        if (col[e] < 0 || col[e] >= ncols) {
+            latch(113, "HIP_MatCreateFromCSROrdered: column index %d out of range", col[e]);
+            return nullptr;
+        }
+    std::vector<int> p((size_t)nrows), ip((size_t)nrows, -1);
+    if (perm) {
+        std::copy(perm, perm + nrows, p.begin());
+    } else if (HIP_MatLocalityOrder(nrows, rowptr, col, 512, p.data()) != 0) {
+        latch(117, "HIP_MatCreateFromCSROrdered: ordering failed");
+        return nullptr;
+    }
+    for (int i = 0; i < nrows; i++) {
+        if (p[i] < 0 || p[i] >= nrows || ip[p[i]] != -1) {
+            latch(116, "HIP_MatCreateFromCSROrdered: perm is not a permutation (entry %d)", i);
+            return nullptr;
+        }
+        ip[p[i]] = i;
+    }
+    // row i of P A P^T is the caller's row p[i], its entries in their order,
+    // columns renumbered
+    std::vector<int> rp((size_t)nrows + 1, 0), cj((size_t)std::max(1, nnz)), nzmap((size_t)std::max(1, nnz));
+    std::vector<double> v((size_t)std::max(1, nnz));
+    for (int i = 0; i < nrows; i++) {
+        const int o = p[i];
+        int q = rp[i];
+        for (int e = rowptr[o]; e < rowptr[o + 1]; e++, q++) {
+            cj[q] = ip[col[e]];
+            v[q] = val[e];
+            nzmap[e] = q;
+        }
+        rp[i + 1] = q;
+    }
+    HIP_MAT *A = HIP_MatCreateFromCSR(nrows, ncols, nnz, rp.data(), cj.data(), v.data());
+    if (!A) return nullptr;
+    A->perm_h = new int[nrows];
+    A->iperm_h = new int[nrows];
+    A->nzmap_h = new int[std::max(1, nnz)];
+    std::copy(p.begin(), p.end(), A->perm_h);
+    std::copy(ip.begin(), ip.end(), A->iperm_h);
+    std::copy(nzmap.begin(), nzmap.begin() + std::max(1, nnz), A->nzmap_h);
+    return A;
+}
+
+extern "C" GCGE_INT HIP_MatGetOrder(const HIP_MAT *mat, GCGE_INT *perm)
+{
+    if (!mat || mat->magic != kMagicMat || !perm) return -1;
+    if (!mat->perm_h) return 0;
+    std::copy(mat->perm_h, mat->perm_h + mat->n, perm);
+    return 1;
+}
+
 extern "C" void HIP_MatDestroy(HIP_MAT **mat)
 {
     if (!mat || !*mat) return;
@@ -725,6 +790,9 @@ extern "C" void HIP_MatDestroy(HIP_MAT **mat)
         if (A->halo_ptr) (void)hipFree(A->halo_ptr);
         if (A->halo_col) (void)hipFree(A->halo_col);
         if (A->lcol) (void)hipFree(A->lcol);
+        delete[] A->perm_h;
+        delete[] A->iperm_h;
+        delete[] A->nzmap_h;
         g_ctx.dev_bytes -= sizeof(int) * (double)(A->n + 1) + 12.0 * A->nnz + (A->val32 ? 4.0 * A->nnz : 0.0) +
                            (A->rowidx ? 4.0 * A->nnz : 0.0) + A->halo_bytes;
         A->magic = 0;
@@ -741,6 +809,12 @@ extern "C" GCGE_INT HIP_MatSetValues(HIP_MAT *A, const GCGE_DOUBLE *val)
     if (!A || A->magic != kMagicMat || !val) {
         latch(114, "HIP_MatSetValues: bad arguments");
         return 114;
+    }
+    std::vector<double> mapped;
+    if (A->nzmap_h) {  // the caller's entry order -> the stored order
+        mapped.resize((size_t)A->nnz);
+        for (int e = 0; e < A->nnz; e++) mapped[A->nzmap_h[e]] = val[e];
+        val = mapped.data();
     }
     if (!sync()) return g_err;  // kernels queued on the backend stream may still read the old values
     if (!check(hipMemcpy(A->val64, val, sizeof(double) * A->nnz, hipMemcpyHostToDevice), "H2D val")) return g_err;
@@ -791,12 +865,20 @@ extern "C" void HIP_MultiVecDestroy(void ***mv)
     *mv = nullptr;
 }
 
+// the caller's row order of a multivector: that of the matrix it was created from
+static const int *mv_perm(const MV *m) { return (m->mat && m->mat->perm_h) ? m->mat->perm_h : nullptr; }
+
 template <typename T>
 static void host_to_dev_cols(MV *m, int start, int ncols, const double *host, int ldh)
 {
     std::vector<T> buf((size_t)m->n);
+    const int *perm = mv_perm(m);
     for (int j = 0; j < ncols && !failed(); j++) {
-        for (int64_t i = 0; i < m->n; i++) buf[i] = (T)host[(size_t)j * ldh + i];
+        const double *hc = host + (size_t)j * ldh;
+        if (perm)
+            for (int64_t i = 0; i < m->n; i++) buf[i] = (T)hc[perm[i]];
+        else
+            for (int64_t i = 0; i < m->n; i++) buf[i] = (T)hc[i];
         check(hipMemcpy(col_ptr(m, start + j), buf.data(), sizeof(T) * m->n, hipMemcpyHostToDevice), "H2D column");
     }
 }
@@ -811,7 +893,10 @@ extern "C" GCGE_INT HIP_MultiVecSetFromHost(void **mv, GCGE_INT start, GCGE_INT 
         return g_err;
     }
     if (!sync()) return g_err;
-    if (m->prec == GCGE_HIP_FP64) {
+    if (mv_perm(m)) {
+        if (m->prec == GCGE_HIP_FP64) host_to_dev_cols<double>(m, start, ncols, host, ldh);
+        else host_to_dev_cols<float>(m, start, ncols, host, ldh);
+    } else if (m->prec == GCGE_HIP_FP64) {
         if (ldh == m->ld) {
             check(hipMemcpy(col_ptr(m, start), host, sizeof(double) * (size_t)m->ld * ncols, hipMemcpyHostToDevice),
                   "H2D block");
@@ -836,17 +921,31 @@ extern "C" GCGE_INT HIP_MultiVecGetToHost(void **mv, GCGE_INT start, GCGE_INT nc
         return g_err;
     }
     if (!sync()) return g_err;
-    if (m->prec == GCGE_HIP_FP64) {
+    const int *perm = mv_perm(m);
+    if (m->prec == GCGE_HIP_FP64 && !perm) {
         check(hipMemcpy2D(host, sizeof(double) * ldh, col_ptr(m, start), sizeof(double) * m->ld,
                           sizeof(double) * m->n, ncols, hipMemcpyDeviceToHost),
               "D2H 2D");
+    } else if (m->prec == GCGE_HIP_FP64) {
+        std::vector<double> buf((size_t)m->n);
+        for (int j = 0; j < ncols && !failed(); j++) {
+            if (!check(hipMemcpy(buf.data(), col_ptr(m, start + j), sizeof(double) * m->n, hipMemcpyDeviceToHost),
+                       "D2H column"))
+                break;
+            double *hc = host + (size_t)j * ldh;
+            for (int64_t i = 0; i < m->n; i++) hc[perm[i]] = buf[i];
+        }
     } else {
         std::vector<float> buf((size_t)m->n);
         for (int j = 0; j < ncols && !failed(); j++) {
             if (!check(hipMemcpy(buf.data(), col_ptr(m, start + j), sizeof(float) * m->n, hipMemcpyDeviceToHost),
                        "D2H column"))
                 break;
-            for (int64_t i = 0; i < m->n; i++) host[(size_t)j * ldh + i] = buf[i];
+            double *hc = host + (size_t)j * ldh;
+            if (perm)
+                for (int64_t i = 0; i < m->n; i++) hc[perm[i]] = buf[i];
+            else
+                for (int64_t i = 0; i < m->n; i++) hc[i] = buf[i];
         }
     }
     return g_err;
@@ -1432,6 +1531,10 @@ extern "C" GCGE_SOLVER *GCGE_HIP_Solver_Init(HIP_MAT *A, HIP_MAT *B, GCGE_INT nu
         latch(170, "GCGE_HIP_Solver_Init: bad matrices");
         return nullptr;
     }
+    if (B && (!A->perm_h != !B->perm_h || (A->perm_h && !std::equal(A->perm_h, A->perm_h + A->n, B->perm_h)))) {
+        latch(171, "GCGE_HIP_Solver_Init: A and B were created with different row orders");
+        return nullptr;
+    }
     GCGE_SOLVER *solver;
     GCGE_SOLVER_Create(&solver);
     if (num_eigenvalues != -1) solver->para->nev = num_eigenvalues;
@@ -1558,6 +1661,12 @@ bool gradient_term(HIP_MAT *M, MV *mx, GCGE_INT k, const double *wgt, GCGE_INT n
                    const GCGE_INT *jnz, const GCGE_DOUBLE *jval, double scale, bool accumulate, double *dgrad)
 {
     const int nj = jrowptr[nparam];
+    std::vector<int> mapped;
+    if (M->nzmap_h && nj > 0) {  // the caller's entry indices -> the stored order
+        mapped.resize((size_t)nj);
+        for (int t = 0; t < nj; t++) mapped[t] = M->nzmap_h[jnz[t]];
+        jnz = mapped.data();
+    }
     double *g = nullptr, *dval = nullptr;
     int *drp = nullptr, *dnz = nullptr;
     bool ok = check(hipMalloc((void **)&g, sizeof(double) * std::max(1, M->nnz)), "hipMalloc g") &&

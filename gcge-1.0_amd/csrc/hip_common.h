@@ -76,6 +76,12 @@ struct HIP_MAT_ {
     int *halo_col;            // device: the other rows a block's entries read, ascending
     unsigned short *lcol;     // device, nnz: each entry's slot (own row r0+i -> i, extra j -> 256+j)
     double halo_bytes;        // device bytes of the three arrays
+    // row order (HIP_MatCreateFromCSROrdered): the device matrix is
+    // P A P^T; multivectors created from it hold rows in the same order and
+    // are permuted at host transfers.  Host arrays, null when unordered.
+    int *perm_h;              // n: new position -> the caller's row
+    int *iperm_h;             // n: the caller's row -> new position
+    int *nzmap_h;             // nnz: the caller's entry index -> stored entry index
 };
 
 // largest LDS slot count per column of a k_spmm_halo row block

@@ -386,6 +386,12 @@ __global__ __launch_bounds__(kSpmmRows) void k_spmm_long(const int *__restrict__
 // range of rows (a stencil's plane neighbours stay in its L2).
 constexpr int kHaloMaxE = 8;
 
+// KC consecutive values of one LDS slot, moved as one vector
+template <typename T, int KC>
+struct alignas(KC * sizeof(T) >= 16 ? 16 : KC * sizeof(T)) SlotVec {
+    T v[KC];
+};
+
 template <typename T, bool DOT, int KC>
 __global__ __launch_bounds__(kSpmmRows) void k_spmm_halo(int n, const int *__restrict__ rowptr,
                                                          const unsigned short *__restrict__ lcol,
@@ -395,8 +401,9 @@ __global__ __launch_bounds__(kSpmmRows) void k_spmm_halo(int n, const int *__res
                                                          ColScalars g, double *__restrict__ partial, int nblk,
                                                          int long_thr, int pstride, int hcap)
 {
+    // s_x holds slot-major KC-column rows: slot j, column cc at s_x[j * KC + cc]
     extern __shared__ __align__(16) unsigned char halo_smem[];
-    T *s_x = reinterpret_cast<T *>(halo_smem);                    // [KC][hcap]
+    T *s_x = reinterpret_cast<T *>(halo_smem);                    // [hcap][KC]
     int *s_h = reinterpret_cast<int *>(s_x + (size_t)KC * hcap);  // [hcap - kSpmmRows]
     __shared__ double sdot[DOT ? kMaxCols : 1][4];
     const int per = (int)(gridDim.x >> 3);
@@ -421,29 +428,53 @@ __global__ __launch_bounds__(kSpmmRows) void k_spmm_halo(int n, const int *__res
         v[u] = u < cnt ? val[s + u] : T(0);
     }
     const int lane = tid & 63, wid = tid >> 6;
+    using SV = SlotVec<T, KC>;
+    SV *sv = reinterpret_cast<SV *>(s_x);
     __syncthreads();
     for (int c0 = 0; c0 < k; c0 += KC) {
+        const T *xc[KC];
 #pragma unroll
         for (int cc = 0; cc < KC; cc++) {
             const int c = min(c0 + cc, k - 1);
-            const T *xc = X + (g.use_idx ? (int64_t)g.idx[c] : (int64_t)c) * ldx;
-            T *sx = s_x + cc * hcap;
-            if (tid < nown) sx[tid] = xc[r0 + tid];
-            for (int j = tid; j < nh; j += kSpmmRows) sx[kSpmmRows + j] = xc[s_h[j]];
+            xc[cc] = X + (g.use_idx ? (int64_t)g.idx[c] : (int64_t)c) * ldx;
+        }
+        if (tid < nown) {
+            SV w;
+#pragma unroll
+            for (int cc = 0; cc < KC; cc++) w.v[cc] = xc[cc][r0 + tid];
+            sv[tid] = w;
+        }
+        for (int j = tid; j < nh; j += kSpmmRows) {
+            const int row = s_h[j];
+            SV w;
+#pragma unroll
+            for (int cc = 0; cc < KC; cc++) w.v[cc] = xc[cc][row];
+            sv[kSpmmRows + j] = w;
         }
         __syncthreads();
+        T acc[KC];
+#pragma unroll
+        for (int cc = 0; cc < KC; cc++) acc[cc] = T(0);
+#pragma unroll
+        for (int u = 0; u < kHaloMaxE; u++)
+            if (u < cnt) {
+                const SV x = sv[lc[u]];
+#pragma unroll
+                for (int cc = 0; cc < KC; cc++) acc[cc] = acc[cc] + v[u] * x.v[cc];
+            }
+        for (int e = s + kHaloMaxE; e < s + cnt; e++) {
+            const SV x = sv[lcol[e]];
+            const T ve = val[e];
+#pragma unroll
+            for (int cc = 0; cc < KC; cc++) acc[cc] = acc[cc] + ve * x.v[cc];
+        }
         double d[KC];
+        const SV xo = sv[tid];  // own row (written by this thread when active)
 #pragma unroll
         for (int cc = 0; cc < KC; cc++) {
-            const T *sx = s_x + cc * hcap;
-            T acc = T(0);
-#pragma unroll
-            for (int u = 0; u < kHaloMaxE; u++)
-                if (u < cnt) acc = acc + v[u] * sx[lc[u]];
-            for (int e = s + kHaloMaxE; e < s + cnt; e++) acc = acc + val[e] * sx[lcol[e]];
             const int c = c0 + cc;
-            if (active && c < k) Y[(int64_t)c * ldy + r] = acc;
-            if (DOT) d[cc] = (active && c < k) ? (double)sx[tid] * (double)acc : 0.0;
+            if (active && c < k) Y[(int64_t)c * ldy + r] = acc[cc];
+            if (DOT) d[cc] = (active && c < k) ? (double)xo.v[cc] * (double)acc[cc] : 0.0;
         }
         if (DOT) {
             const double colsum = wave_colsum<KC>(d, lane);
@@ -1222,7 +1253,8 @@ static int halo_kc(const HIP_MAT *A, size_t elem)
     const int env_kc = e ? atoi(e) : 0;
     const char *l = getenv("GCGE_HIP_HALO_LDS");
     const long env_lds = l ? atol(l) : 40960;
-    int kc = (env_kc == 1 || env_kc == 2 || env_kc == 4 || env_kc == 8) ? env_kc : 8;
+    // default: 32-byte slots (4 FP64 or 8 FP32 columns per pass) within the budget
+    int kc = (env_kc == 1 || env_kc == 2 || env_kc == 4 || env_kc == 8) ? env_kc : (elem == 8 ? 4 : 8);
     if (env_kc != kc)
         while (kc > 1 && (size_t)kc * A->halo_cap * elem > (size_t)env_lds) kc /= 2;
     // never beyond 56 KiB of dynamic LDS (the kernel adds 4 KiB of its own)

@@ -40,21 +40,46 @@ def _check(what):
     capi.raise_on_error(what)
 
 
-class Matrix:
-    """A CSR matrix uploaded to the device (HIP_MAT)."""
+def locality_order(M, leaf=0):
+    """HIP_MatLocalityOrder of a CSR pattern: perm[new position] = row."""
+    rp = np.ascontiguousarray(M.indptr, dtype=np.int32)
+    ci = np.ascontiguousarray(M.indices, dtype=np.int32)
+    perm = np.zeros(M.shape[0], dtype=np.int32)
+    rc = capi.load().HIP_MatLocalityOrder(M.shape[0], _ip(rp), _ip(ci), leaf, _ip(perm))
+    if rc != 0:
+        raise ValueError(f"HIP_MatLocalityOrder: bad pattern ({rc})")
+    return perm
 
-    def __init__(self, M):
+
+class Matrix:
+    """A CSR matrix uploaded to the device (HIP_MAT).  order: None (the
+    caller's row order), "locality" (HIP_MatLocalityOrder) or a permutation
+    perm[new position] = row (HIP_MatCreateFromCSROrdered); multivectors
+    created from the matrix then transfer in the caller's order."""
+
+    def __init__(self, M, order=None):
         L = capi.load()
         self.n = M.shape[0]
         self.rowptr = np.ascontiguousarray(M.indptr, dtype=np.int32)
         self.col = np.ascontiguousarray(M.indices, dtype=np.int32)
         self.val = np.ascontiguousarray(M.data, dtype=np.float64)
         self.nnz = int(self.col.size)
-        self.ptr = L.HIP_MatCreateFromCSR(self.n, M.shape[1], self.nnz, _ip(self.rowptr), _ip(self.col),
-                                          _dp(self.val))
+        if order is None:
+            self.ptr = L.HIP_MatCreateFromCSR(self.n, M.shape[1], self.nnz, _ip(self.rowptr), _ip(self.col),
+                                              _dp(self.val))
+        else:
+            perm = None if isinstance(order, str) and order == "locality" else \
+                np.ascontiguousarray(order, dtype=np.int32)
+            self.ptr = L.HIP_MatCreateFromCSROrdered(self.n, M.shape[1], self.nnz, _ip(self.rowptr), _ip(self.col),
+                                                     _dp(self.val), _ip(perm) if perm is not None else None)
         if not self.ptr:
             _check("HIP_MatCreateFromCSR")
             raise RuntimeError("HIP_MatCreateFromCSR failed")
+
+    def order(self):
+        """The row order (perm[new position] = row), or None."""
+        perm = np.zeros(self.n, dtype=np.int32)
+        return perm if capi.load().HIP_MatGetOrder(self.ptr, _ip(perm)) == 1 else None
 
     @classmethod
     def from_file(cls, path):

@@ -168,6 +168,9 @@ SIGNATURES = {
     "HIP_MatCreateFromCSR": (c_void_p, [c_int, c_int, c_int, IP, IP, DP]),
     "HIP_ReadMatFile": (c_void_p, [c_char_p]),
     "HIP_WriteMatFile": (c_int, [c_char_p, c_int, c_int, c_int, IP, IP, DP, c_int]),
+    "HIP_MatLocalityOrder": (c_int, [c_int, IP, IP, c_int, IP]),
+    "HIP_MatCreateFromCSROrdered": (c_void_p, [c_int, c_int, c_int, IP, IP, DP, IP]),
+    "HIP_MatGetOrder": (c_int, [c_void_p, IP]),
     "HIP_MatDestroy": (None, [C.POINTER(c_void_p)]),
     "HIP_MatNumRows": (c_int, [c_void_p]),
     "HIP_MatNumNonzeros": (c_int, [c_void_p]),

@@ -42,6 +42,28 @@ HIP_MAT *HIP_MatCreateFromCSR(GCGE_INT nrows, GCGE_INT ncols, GCGE_INT nnz, cons
  * nnz, 0, then rowptr[nrow+1], col[nnz] (int32) and val[nnz] (FP64), little
  * endian, as HIP_WriteMatFile writes it. */
 HIP_MAT *HIP_ReadMatFile(const char *filename);
+/* A locality order of the rows of a square CSR pattern (host only, no
+ * coordinates needed): perm[new position] = row, built by recursive halving
+ * of the matrix graph in breadth-first order, parts of at most leaf rows
+ * (0: 512) in breadth-first order.  On a grid it groups rows into compact
+ * pieces, so a 256-row block reads its own rows plus about as many again
+ * (its surface) instead of whole neighbouring planes.  Returns 0, or
+ * non-zero for a bad pattern. */
+GCGE_INT HIP_MatLocalityOrder(GCGE_INT nrows, const GCGE_INT *rowptr, const GCGE_INT *col, GCGE_INT leaf,
+                              GCGE_INT *perm);
+/* Upload P A P^T for the row order perm (perm[new position] = the caller's
+ * row; NULL: HIP_MatLocalityOrder with leaf 0).  Each row keeps its entries'
+ * order, so products are bit-identical row by row to A's.  Multivectors
+ * created from the matrix hold rows in this order: HIP_MultiVecSetFromHost,
+ * HIP_MultiVecGetToHost and the random initial vectors use the caller's
+ * order, HIP_MatSetValues and the gradient Jacobians the caller's entry
+ * order; A and B of one solver must share the order (error 171 otherwise).
+ * MultiVecPrint prints the stored order. */
+HIP_MAT *HIP_MatCreateFromCSROrdered(GCGE_INT nrows, GCGE_INT ncols, GCGE_INT nnz, const GCGE_INT *rowptr,
+                                     const GCGE_INT *col, const GCGE_DOUBLE *val, const GCGE_INT *perm);
+/* Writes the matrix's row order into perm[n] and returns 1, or returns 0
+ * (natural order, perm untouched); -1 for a bad matrix. */
+GCGE_INT HIP_MatGetOrder(const HIP_MAT *mat, GCGE_INT *perm);
 /* Write CSR arrays (host) in the text format (binary = 0; values with 17
  * significant digits, so they read back exactly) or the binary one. */
 GCGE_INT HIP_WriteMatFile(const char *filename, GCGE_INT nrows, GCGE_INT ncols, GCGE_INT nnz,

@@ -46,6 +46,8 @@ extern void dsymm_(const char *side, const char *uplo, int *m, int *n,
 extern double ddot_(int *n, double *x, int *incx, double *y, int *incy);
 extern double dnrm2_(int *n, double *x, int *incx);
 extern void dscal_(int *n, double *a, double *x, int *incx);
+extern void daxpy_(int *n, double *a, double *x, int *incx, double *y, int *incy);
+extern void dcopy_(int *n, double *x, int *incx, double *y, int *incy);
 
 /* ------------------------------------------------------------------------ */
 /* CSR backend (app/csr/gcge_app_csr.c)                                      */
@@ -491,6 +493,132 @@ static void multi_orth_insub(S_t *S, double *Vd, int ldV, int nrows, int start, 
     multi_orth_insub(S, Vd, ldV, nrows, mid, end);
 }
 
+/* The dense array operations of src/gcge_ops.c:361-429 (BLAS-1). */
+static double arr_norm(double *x, int n)
+{
+    int inc = 1;
+    return dnrm2_(&n, x, &inc);
+}
+
+static double arr_dot(double *x, double *y, int n)
+{
+    int inc = 1;
+    return ddot_(&n, x, &inc, y, &inc);
+}
+
+static void arr_axpby(double a, double *x, double b, double *y, int n)
+{
+    int inc = 1;
+    if (b != 1.0) dscal_(&n, &b, y, &inc);
+    daxpy_(&n, &a, x, &inc, y, &inc);
+}
+
+static void arr_copy(double *x, double *y, int n)
+{
+    int inc = 1;
+    dcopy_(&n, x, &inc, y, &inc);
+}
+
+static void arr_scale(double a, double *x, int n)
+{
+    int inc = 1;
+    dscal_(&n, &a, x, &inc);
+}
+
+/* GCGE_OrthonormalizationInSubspace, gcge_orthogonal.c:581-645
+ * (p_orth_type "gs" and any name the reference does not list). */
+static void gs_insub(S_t *S, double *Vd, int ldV, int nrows, int start, int *end)
+{
+    int current, idx, reorth_count;
+    double vin, vout, ratio, ip;
+    for (current = start; current < *end; ++current) {
+        double *vc = Vd + (size_t)current * ldV;
+        vout = arr_norm(vc, nrows);
+        if (current != 0) {
+            reorth_count = 0;
+            do {
+                vin = vout;
+                for (idx = 0; idx < current; idx++) {
+                    ip = arr_dot(Vd + (size_t)idx * ldV, vc, nrows);
+                    arr_axpby(-ip, Vd + (size_t)idx * ldV, 1.0, vc, nrows);
+                }
+                vout = arr_norm(vc, nrows);
+                ratio = vout / vin;
+                reorth_count++;
+            } while (ratio < S->P.reorth_tol && reorth_count < S->P.max_reorth_time && vout > S->P.orth_zero_tol);
+        }
+        if (vout > S->P.orth_zero_tol) {
+            arr_scale(1.0 / vout, vc, nrows);
+        } else {
+            if (current != *end - 1) arr_copy(Vd + (size_t)(*end - 1) * ldV, vc, nrows);
+            (*end)--;
+            current--;
+        }
+    }
+}
+
+/* the self sweep of GCGE_BOrthonormalizationInSubspace (:1538-1575) and
+ * GCGE_SCBOrthonormalizationInSubspace (:1674-1714) */
+static void self_insub(S_t *S, double *Vd, int ldV, int nrows, int start, int *end)
+{
+    int current, j, len, one = 1;
+    double alpha = 1.0, beta = 0.0, norm, tol = S->P.orth_zero_tol, *d = S->subspace_dtmp;
+    for (current = start; current < *end; ++current) {
+        double *vc = Vd + (size_t)current * ldV;
+        len = *end - current;
+        dgemm_("T", "N", &len, &one, &nrows, &alpha, vc, &ldV, vc, &ldV, &beta, d, &len);
+        norm = sqrt(d[0]);
+        if (norm > tol) {
+            arr_scale(1.0 / norm, vc, nrows);
+            for (j = current + 1; j < *end; j++)
+                if (fabs(d[j - current] / norm) > 10.0 * tol)
+                    arr_axpby(-d[j - current] / norm, vc, 1.0, Vd + (size_t)j * ldV, nrows);
+        } else {
+            if (current != *end - 1) arr_copy(Vd + (size_t)(*end - 1) * ldV, vc, nrows);
+            (*end)--;
+            current--;
+        }
+    }
+}
+
+/* GCGE_BOrthonormalizationInSubspace, gcge_orthogonal.c:1501-1578. */
+static void bgs_insub(S_t *S, double *Vd, int ldV, int nrows, int start, int *end)
+{
+    int pass, current, j, len, one = 1;
+    double alpha = 1.0, beta = 0.0, tol = S->P.orth_zero_tol, *d = S->subspace_dtmp;
+    for (pass = 0; pass < 2; pass++) {
+        len = *end - start;
+        for (current = 0; current < start; ++current) {
+            dgemm_("T", "N", &len, &one, &nrows, &alpha, Vd + (size_t)start * ldV, &ldV, Vd + (size_t)current * ldV,
+                   &ldV, &beta, d, &len);
+            for (j = start; j < *end; j++)
+                if (fabs(d[j - start]) > 10.0 * tol)
+                    arr_axpby(-d[j - start], Vd + (size_t)current * ldV, 1.0, Vd + (size_t)j * ldV, nrows);
+        }
+        self_insub(S, Vd, ldV, nrows, start, end);
+    }
+}
+
+/* GCGE_SCBOrthonormalizationInSubspace, gcge_orthogonal.c:1619-1718. */
+static void scbgs_insub(S_t *S, double *Vd, int ldV, int nrows, int start, int *end)
+{
+    int rc, len;
+    double alpha, beta, *d = S->subspace_dtmp;
+    if (start != 0) {
+        for (rc = 0; rc < S->P.max_reorth_time; rc++) {
+            len = *end - start;
+            alpha = 1.0;
+            beta = 0.0;
+            dgemm_("T", "N", &start, &len, &nrows, &alpha, Vd, &ldV, Vd + (size_t)start * ldV, &ldV, &beta, d,
+                   &start);
+            alpha = -1.0;
+            beta = 1.0;
+            dgemm_("N", "N", &ldV, &len, &start, &alpha, Vd, &ldV, d, &start, &beta, Vd + (size_t)start * ldV, &ldV);
+        }
+    }
+    self_insub(S, Vd, ldV, nrows, start, end);
+}
+
 /* GCGE_StableMultiOrthonormalizationInSubspace, gcge_orthogonal.c:2215-2247. */
 static void stable_multi_orth_insub(S_t *S, double *Vd, int ldV, int nrows, int start, int *end)
 {
@@ -927,8 +1055,15 @@ static void compute_p(S_t *S, double *coef, double **V)
     orth_length = ldc - orth_start;
     p_start -= orth_start;
     p_end = p_start + p_ncols;
-    stable_multi_orth_insub(S, coef + (size_t)orth_start * ldc + orth_start, ldc, orth_length,
-                            p_start, &p_end);
+    if (strcmp(S->P.p_orth_type, "scbgs") == 0)
+        scbgs_insub(S, coef + (size_t)orth_start * ldc + orth_start, ldc, orth_length, p_start, &p_end);
+    else if (strcmp(S->P.p_orth_type, "bgs") == 0)
+        bgs_insub(S, coef + (size_t)orth_start * ldc + orth_start, ldc, orth_length, p_start, &p_end);
+    else if (strcmp(S->P.p_orth_type, "multi") == 0)
+        stable_multi_orth_insub(S, coef + (size_t)orth_start * ldc + orth_start, ldc, orth_length,
+                                p_start, &p_end);
+    else
+        gs_insub(S, coef + (size_t)orth_start * ldc + orth_start, ldc, orth_length, p_start, &p_end);
     p_ncols = p_end - p_start;
     p_end += orth_start;
     p_end_in_V_tmp = p_end - nev;
@@ -1068,7 +1203,6 @@ int orc_solve(const orc_csr *A, const orc_csr *B, const orc_params *p, const dou
     nev = S->P.nev;
     if (nev <= 0 || nev > n) return -1;
     if (strcmp(S->P.orth_type, "B") != 0) return -2;          /* "A" not restated */
-    if (strcmp(S->P.p_orth_type, "multi") != 0) return -4;
 
     /* GCGE_PARA_Setup, gcge_para.c:357-386 */
     S->num_unlock = nev;

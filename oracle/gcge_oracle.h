@@ -44,7 +44,7 @@ typedef struct {
     const char *conv_type;      /* "R", "O", "A", or the extension "NS"      */
     const char *orth_type;      /* "B" (only B-orthogonalization is restated) */
     const char *w_orth_type;    /* "multi", "gs", "bgs", "cbgs" or "scbgs"   */
-    const char *p_orth_type;    /* "multi"                                   */
+    const char *p_orth_type;    /* "multi", "gs", "bgs" or "scbgs"           */
     double conv_omega_norm;
     double orth_zero_tol;       /* DBL_EPSILON is added in setup             */
     double reorth_tol;

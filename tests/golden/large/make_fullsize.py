@@ -12,7 +12,8 @@ Both start from X0 = problems.random_x0(n, nev, seed) and use the
 reference's relative criterion "R" (||r|| / ||x|| / max(1, |lambda|)) with
 ev_tol = 1e-8 * 0.95 * lambda_1: with B = I and every wanted eigenvalue below
 1, R <= ev_tol implies ||A x - lambda x|| / (|lambda| ||x||) <= 1e-8 for every
-pair.  lambda_1 comes from scipy's shift-invert Lanczos before the run.  The
+pair.  The generalized case (lap3dmass96) keeps the reference's default
+ev_tol 1e-8.  lambda_1 comes from scipy's shift-invert Lanczos before the run.  The
 fixture keeps the generator parameters, a fingerprint of the matrix and the
 reference's outputs (eigenvalues, residuals, iteration count, per-iteration
 history); the test regenerates matrix and X0.
@@ -39,13 +40,19 @@ from gcge_hip import problems as P  # noqa: E402
 CASES = {
     "lap2d128": dict(gen="lap2d", m=128, gen_seed=0, nev=20, x0_seed=5),
     "vardiff2d512": dict(gen="vardiff2d", m=512, gen_seed=0, nev=16, x0_seed=6),
+    # the generalized workload: 3-D Laplacian with the P1 mass matrix on 96^3
+    # (n = 884,736), lowest 50 pairs, the reference's default ev_tol 1e-8 on
+    # its "R" criterion (about 35 minutes on one core)
+    "lap3dmass96": dict(gen="lap3d_mass", m=96, gen_seed=0, nev=50, x0_seed=7, ev_tol=1e-8),
 }
 
 
 def matrix(case):
     if case["gen"] == "lap2d":
-        return P.lap2d(case["m"])[0]
-    return P.vardiff2d(case["m"], seed=case["gen_seed"])[0]
+        return P.lap2d(case["m"])
+    if case["gen"] == "lap3d_mass":
+        return P.lap3d_mass(case["m"])
+    return P.vardiff2d(case["m"], seed=case["gen_seed"])
 
 
 def fingerprint(A):
@@ -59,14 +66,17 @@ def main(names):
         O.build_ref()
     for name in names:
         case = CASES[name]
-        A = matrix(case)
+        A, B = matrix(case)
         n, nev = A.shape[0], case["nev"]
-        lam1 = float(sla.eigsh(A.tocsc(), k=1, sigma=0.0, which="LM", return_eigenvectors=False)[0])
-        tol = float("%.3e" % (1e-8 * 0.95 * lam1))
+        if "ev_tol" in case:
+            lam1, tol = float("nan"), case["ev_tol"]
+        else:
+            lam1 = float(sla.eigsh(A.tocsc(), k=1, sigma=0.0, which="LM", return_eigenvectors=False)[0])
+            tol = float("%.3e" % (1e-8 * 0.95 * lam1))
         flags = ["-gcge_ev_max_it", "2000", "-gcge_ev_tol", repr(tol), "-gcge_conv_type", "R",
                  "-gcge_print_para", "0"]
         X0 = P.random_x0(n, nev, case["x0_seed"])
-        r = O.ref_solve(A, None, nev, X0=X0, flags=flags, timeout=7200)
+        r = O.ref_solve(A, B, nev, X0=X0, flags=flags, timeout=7200)
         its, _ = O.parse_trace(r["stdout"])
         ints, dsum = fingerprint(A)
         np.savez_compressed(

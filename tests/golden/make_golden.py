@@ -88,6 +88,12 @@ def cases():
                     ["-gcge_w_orth_type", wt], dict(w_orth_type=wt)))
         out.append((f"lap3dmass8_{wt}", A8, B8, 10, P.random_x0(A8.shape[0], 10, 16), 1, 0,
                     ["-gcge_w_orth_type", wt, "-gcge_ev_max_it", "60"], dict(w_orth_type=wt, ev_max_it=60)))
+    # the in-subspace orthonormalizations of ComputeP's coefficients (gcge_xpw.c:363-385)
+    for pt in ("gs", "bgs", "scbgs"):
+        out.append((f"lap2d16_p{pt}", L16, None, 6, P.random_x0(L16.shape[0], 6, 20), 1, 0,
+                    ["-gcge_p_orth_type", pt], dict(p_orth_type=pt)))
+        out.append((f"lap3dmass8_p{pt}", A8, B8, 10, P.random_x0(A8.shape[0], 10, 21), 1, 0,
+                    ["-gcge_p_orth_type", pt, "-gcge_ev_max_it", "60"], dict(p_orth_type=pt, ev_max_it=60)))
     # scbgs with W in blocks of 2 columns (w_orth_block_size) and two MGS sweeps per block
     out.append(("lap2d32_scbgs_blk", L32, None, 12, P.random_x0(L32.shape[0], 12, 17), 1, 0,
                 ["-gcge_w_orth_type", "scbgs", "-gcge_w_orth_block_size", "2",

@@ -312,3 +312,42 @@ def test_write_mat_file_formats(tmp_path, binary):
     np.testing.assert_array_equal(ia, B.indptr)
     np.testing.assert_array_equal(ja, B.indices)
     np.testing.assert_array_equal(aa, B.data)   # bit for bit
+
+
+def _block_extras(M, perm, R=256):
+    """Distinct rows outside each 256-row block that the block's entries read,
+    for the matrix in the row order perm (perm[new] = row)."""
+    n = M.shape[0]
+    new = np.empty(n, dtype=np.int64)
+    new[perm] = np.arange(n)
+    out = []
+    for b in range(0, n, R):
+        rows = perm[b:b + R]
+        cols = np.concatenate([new[M.indices[M.indptr[r]:M.indptr[r + 1]]] for r in rows])
+        u = np.unique(cols)
+        out.append(np.sum((u < b) | (u >= b + R)))
+    return np.array(out)
+
+
+def test_locality_order_is_a_permutation_with_compact_blocks():
+    """HIP_MatLocalityOrder (host only): a permutation of the rows; on a 3-D
+    7-point grid its 256-row blocks read about their own size in other rows
+    (their surface), against three times that in the natural order (whole
+    neighbouring planes); a graph with isolated vertices and several
+    components is still fully ordered."""
+    from gcge_hip import locality_order
+    from gcge_hip import problems as P
+    A, _ = P.vardiff3d(24, seed=0)
+    perm = locality_order(A)
+    n = A.shape[0]
+    assert np.array_equal(np.sort(perm), np.arange(n))
+    nat = _block_extras(A, np.arange(n)).mean()
+    loc = _block_extras(A, perm).mean()
+    assert loc < 0.6 * nat, (loc, nat)
+    G, _ = P.chung_lu(5000, mean_deg=2.0, seed=3)
+    pg = locality_order(G, leaf=64)
+    assert np.array_equal(np.sort(pg), np.arange(G.shape[0]))
+    with pytest.raises(ValueError):
+        bad = A.copy()
+        bad.indices[0] = n + 5
+        locality_order(bad)

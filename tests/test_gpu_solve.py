@@ -399,11 +399,14 @@ def test_full_rayleigh_ritz_decomposition_matches_reference(name):
 
 
 def test_orth_variant_iteration_histories_match_reference(capfd):
-    """bgs / scbgs (src/gcge_orthogonal.c:352-528, 967-1279) and soft locking
-    (opt_rr_eig_partly, src/gcge_rayleighritz.c:380-394): the per-iteration
+    """bgs / scbgs (src/gcge_orthogonal.c:352-528, 967-1279), soft locking
+    (opt_rr_eig_partly, src/gcge_rayleighritz.c:380-394) and the gs / bgs /
+    scbgs orthonormalization of ComputeP's coefficients (p_orth_type,
+    src/gcge_xpw.c:363-385): the per-iteration
     Ritz values of the device run follow the reference's to 1e-8 (early
     iterates carry the summation-order differences) until both converge."""
-    for name in ("lap2d16_bgs", "lap3dmass8_scbgs", "lap2d32_scbgs_blk", "lap2d32_rrpartly", "lap3dmass8_rrpartly"):
+    for name in ("lap2d16_bgs", "lap3dmass8_scbgs", "lap2d32_scbgs_blk", "lap2d32_rrpartly", "lap3dmass8_rrpartly",
+                 "lap2d16_pgs", "lap3dmass8_pbgs", "lap3dmass8_pscbgs"):
         d, A, B, X0, kw = load_case(os.path.join(os.path.dirname(__file__), "golden", name + ".npz"))
         capfd.readouterr()
         r = solve(A, B, int(d["nev"]), X0=X0, seed=int(d["seed"]), print_eval=1, **kw)
