@@ -79,16 +79,16 @@ def kernel_delta(after, before):
     return out
 
 
-# profiler class -> the kernel template whose instantiations rocprofv3 lists
-# the SpMM and residual classes time either the column-gather kernel or,
-# for matrices with scattered column indices, the row staging plus the
-# row-gather kernel
+# profiler class -> the kernel template whose instantiations rocprofv3 lists;
+# the SpMM classes time the staged-row-block kernel, the direct column
+# gather, or (matrices with scattered column indices) the row staging plus
+# the row-gather kernel, as the matrix's layout says (HIP_MatGatherKind)
 _ROWS = " / k_rows_from_cols + k_spmm_rows<{}, {}, ...>"
 ROCPROF_KERNEL = {
-    "spmm_f64": "k_spmm<double, false, ...>" + _ROWS.format("double", "false"),
-    "spmm_f32": "k_spmm<float, false, ...>" + _ROWS.format("float", "false"),
-    "spmm_dot_f64": "k_spmm<double, true, ...>" + _ROWS.format("double", "true"),
-    "spmm_dot_f32": "k_spmm<float, true, ...>" + _ROWS.format("float", "true"),
+    "spmm_f64": "k_spmm_halo<double, false, ...> / k_spmm<double, false, ...>" + _ROWS.format("double", "false"),
+    "spmm_f32": "k_spmm_halo<float, false, ...> / k_spmm<float, false, ...>" + _ROWS.format("float", "false"),
+    "spmm_dot_f64": "k_spmm_halo<double, true, ...> / k_spmm<double, true, ...>" + _ROWS.format("double", "true"),
+    "spmm_dot_f32": "k_spmm_halo<float, true, ...> / k_spmm<float, true, ...>" + _ROWS.format("float", "true"),
     "axpby": "k_axpby<...>", "dots": "k_dots<...>",
     "cg_update_f64": "k_cg_resid<double> / k_pcg_resid<double>",
     "cg_update_f32": "k_cg_resid<float> / k_pcg_resid<float>",
@@ -209,7 +209,7 @@ def spmm_probe(args):
     from gcge_hip import Matrix, MultiVec, Ops, capi
     L = capi.load()
     A, _ = make_problem(args)
-    mA = Matrix(A)
+    mA = Matrix(A, order=row_order(args))
     ops = Ops()
     out = []
     for prec, w in ((capi.GCGE_HIP_FP64, 8), (capi.GCGE_HIP_FP32, 4)):
@@ -236,6 +236,25 @@ def spmm_probe(args):
             V.close()
             W.close()
     print(json.dumps({"spmm_probe": out, "workload": f"{args.workload} {args.size}"}))
+
+
+def row_order(args):
+    """Row order of the device matrices: "locality" (HIP_MatLocalityOrder:
+    the grids come back in 8x8x8 / 16x16 tiles) for the grid workloads,
+    None (the caller's order) for the random graph, whose scattered columns
+    take the row-gather SpMM, or as --order says."""
+    if args.order == "natural":
+        return None
+    if args.order == "locality":
+        return "locality"
+    return None if args.workload == "chung_lu" else "locality"
+
+
+def spmm_kind(mA):
+    """The SpMM layout the library chose for the matrix (HIP_MatGatherKind)."""
+    from gcge_hip import capi
+    return {0: "columns (k_spmm)", 1: "rows (k_spmm_rows)", 2: "staged row blocks (k_spmm_halo)"}.get(
+        capi.load().HIP_MatGatherKind(mA.ptr), "?")
 
 
 def host_cpu():
@@ -275,8 +294,11 @@ def main():
                          "itself as the timed step (for profiling it; use with --no-fp64 --no-fp64-same)")
     ap.add_argument("--verbose", action="store_true",
                     help="print GCGE's per-iteration convergence line (stdout, before the JSON line)")
-    ap.add_argument("--stagnation-window", type=int, default=10)
+    ap.add_argument("--stagnation-window", type=int, default=2)
+    ap.add_argument("--stagnation-ratio", type=float, default=0.5)
     ap.add_argument("--stagnation-start", type=float, default=1e-3)
+    ap.add_argument("--order", default="auto", choices=["auto", "locality", "natural"],
+                    help="row order of the device matrices (auto: locality for the grid workloads)")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 solve (reference defaults)")
     ap.add_argument("--cg-max-it", type=int, default=30,
                     help="CG step cap of the mixed path's inner solves (the FP64 drop-in keeps the reference's 20)")
@@ -322,15 +344,16 @@ def main():
     X0 = P.random_x0(n, nev, args.seed + 1)
     log(f"problem {args.workload} size {args.size}: n={n} nnz={nnz} nev={nev} ({time.perf_counter() - t:.1f}s)")
     t = time.perf_counter()
-    mA = Matrix(A)
-    mB = Matrix(B) if B is not None else None
+    order = row_order(args)
+    mA = Matrix(A, order=order)
+    mB = Matrix(B, order=mA.order()) if B is not None else None
     X0 = MultiVec(mA, nev, data=X0)  # resident in HBM before any timed solve
     L.GCGE_HIP_Synchronize()
     log(f"upload of the matrix and X0 over PCIe: {time.perf_counter() - t:.2f}s")
 
     common = dict(conv_type="NS", ev_tol=args.tol, ev_max_it=args.max_it, print_result=1 if args.verbose else 0)
     mixed = dict(fp32_tol=args.fp32_tol, fp32_max_it=args.max_it, fp64_max_it=args.max_it,
-                 stagnation_window=args.stagnation_window, stagnation_ratio=0.7,
+                 stagnation_window=args.stagnation_window, stagnation_ratio=args.stagnation_ratio,
                  stagnation_start=args.stagnation_start, cg_max_it=args.cg_max_it, bcg_precond=args.bcg_precond)
 
     def ns_of(solver, res):
@@ -450,7 +473,9 @@ def main():
                    "mode": args.mode, "fp32_tol": args.fp32_tol, "cg_max_it": args.cg_max_it,
                    "bcg_precond": "jacobi" if args.bcg_precond else "none",
                    "rr_eig": "dsyevd" if args.mode == "mixed" else "dsyevx",
-                   "stagnation": [args.stagnation_window, 0.7, args.stagnation_start]},
+                   "stagnation": [args.stagnation_window, args.stagnation_ratio, args.stagnation_start],
+                   "row_order": "locality (HIP_MatLocalityOrder)" if order else "natural",
+                   "spmm": spmm_kind(mA)},
         "converged": bool(converged),
         "max_ns_residual": float(np.max(ns)),
         "step_seconds": [round(x, 3) for x in step_s],

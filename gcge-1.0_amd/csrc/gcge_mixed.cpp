@@ -96,8 +96,11 @@ extern "C" GCGE_INT GCGE_HIP_SOLVER_SolveMixed(GCGE_SOLVER *solver, const GCGE_H
     // stopped at its stagnation; refinement with CholQR and FP32 inner solves;
     // Jacobi-preconditioned inner solves of up to 30 CG steps in both phases;
     // Rayleigh-Ritz pairs from the full dsyevd decomposition)
-    o.stagnation_window = opts ? opts->stagnation_window : 10;
-    o.stagnation_ratio = (opts && opts->stagnation_ratio > 0) ? opts->stagnation_ratio : 0.7;
+    // the phase stops once max_res has not halved within 2 iterations (on the
+    // five benchmark workloads as fast as or faster than a 10-iteration
+    // window at 0.7: the FP32 floor is reached within an iteration or two)
+    o.stagnation_window = opts ? opts->stagnation_window : 2;
+    o.stagnation_ratio = (opts && opts->stagnation_ratio > 0) ? opts->stagnation_ratio : 0.5;
     o.fp32_w_orth = opts ? opts->fp32_w_orth : 1;
     o.refine_w_orth = opts ? opts->refine_w_orth : 1;
     o.refine_bcg_precision = opts ? opts->refine_bcg_precision : 1;

@@ -463,13 +463,18 @@ extern "C" HIP_MAT *HIP_MatCreateFromCSR(GCGE_INT nrows, GCGE_INT ncols, GCGE_IN
 
 // Staged row blocks of k_spmm_halo: for each 256-row block, the rows its
 // entries read besides its own (ascending), and every entry's LDS slot.
-// Built with GCGE_HIP_SPMM_GATHER=halo when no block reads more than
-// kHaloCap - 256 other rows; rows longer than kLongRow (k_spmm_long) are
-// left out.
+// Chosen for the column-gather layout when the blocks are compact: on
+// average a block reads at most as many other rows as it owns, and no block
+// more than kHaloCap - 256 (a grid in tile order reads its surface, about
+// 0.9 times its rows; the natural order of a 3-D grid reads whole planes,
+// 2.4 times, where the direct gather of k_spmm is as fast).
+// GCGE_HIP_SPMM_GATHER=halo takes it whenever it fits, =columns or =rows
+// never.  Rows longer than kLongRow (k_spmm_long) are left out.
 static bool build_halo(HIP_MAT *A, const int *rowptr, const int *col)
 {
     const char *e = getenv("GCGE_HIP_SPMM_GATHER");
-    if (!e || strcmp(e, "halo")) return true;  // measured slower than k_spmm on the 3-D stencil: opt-in
+    const bool forced = e && !strcmp(e, "halo");
+    if (e && !forced) return true;
     const int n = A->n, R = 256, nblk = (n + R - 1) / R;
     std::vector<int> hptr((size_t)nblk + 1, 0), hcol, ext;
     std::vector<unsigned short> lcol((size_t)std::max(1, A->nnz), 0);
@@ -497,6 +502,7 @@ static bool build_halo(HIP_MAT *A, const int *rowptr, const int *col)
         }
         hcol.insert(hcol.end(), ext.begin(), ext.end());
         hptr[b + 1] = (int)hcol.size();
+        if (!forced && (int64_t)hcol.size() > (int64_t)n) return true;  // not compact: keep k_spmm
     }
     const size_t nh = std::max<size_t>(1, hcol.size());
     bool ok = check(hipMalloc((void **)&A->halo_ptr, sizeof(int) * hptr.size()), "hipMalloc halo_ptr") &&
@@ -771,6 +777,12 @@ extern "C" GCGE_INT HIP_MatGetOrder(const HIP_MAT *mat, GCGE_INT *perm)
     if (!mat->perm_h) return 0;
     std::copy(mat->perm_h, mat->perm_h + mat->n, perm);
     return 1;
+}
+
+extern "C" GCGE_INT HIP_MatGatherKind(const HIP_MAT *mat)
+{
+    if (!mat || mat->magic != kMagicMat) return -1;
+    return mat->gather_rows ? 1 : mat->halo ? 2 : 0;
 }
 
 extern "C" void HIP_MatDestroy(HIP_MAT **mat)

@@ -1244,17 +1244,19 @@ static bool spmm_rows(HIP_MAT *A, const T *val, const T *Xc, int64_t ldx, ColSca
     return spmm_rows_g<T, DOT, 2>(A, val, Xc, ldx, g, Yc, ldy, kc, out);
 }
 
-// Columns per LDS pass of k_spmm_halo: the largest power of two up to 8
-// whose staged rows fit the byte budget (GCGE_HIP_HALO_LDS, default 40 KiB),
-// or GCGE_HIP_HALO_KC.
+// Columns per LDS pass of k_spmm_halo: 4, or GCGE_HIP_HALO_KC (1, 2, 4 or
+// 8), halved while the staged rows exceed the byte budget
+// (GCGE_HIP_HALO_LDS, default 40 KiB).
 static int halo_kc(const HIP_MAT *A, size_t elem)
 {
     const char *e = getenv("GCGE_HIP_HALO_KC");
     const int env_kc = e ? atoi(e) : 0;
     const char *l = getenv("GCGE_HIP_HALO_LDS");
     const long env_lds = l ? atol(l) : 40960;
-    // default: 32-byte slots (4 FP64 or 8 FP32 columns per pass) within the budget
-    int kc = (env_kc == 1 || env_kc == 2 || env_kc == 4 || env_kc == 8) ? env_kc : (elem == 8 ? 4 : 8);
+    // default: 4 columns per pass in both precisions (measured on 3-D and
+    // 2-D grids in tile order: 8 FP32 columns were 5-10 % slower, 2 were
+    // 10-25 % slower), halved while over the budget
+    int kc = (env_kc == 1 || env_kc == 2 || env_kc == 4 || env_kc == 8) ? env_kc : 4;
     if (env_kc != kc)
         while (kc > 1 && (size_t)kc * A->halo_cap * elem > (size_t)env_lds) kc /= 2;
     // never beyond 56 KiB of dynamic LDS (the kernel adds 4 KiB of its own)

@@ -171,6 +171,7 @@ SIGNATURES = {
     "HIP_MatLocalityOrder": (c_int, [c_int, IP, IP, c_int, IP]),
     "HIP_MatCreateFromCSROrdered": (c_void_p, [c_int, c_int, c_int, IP, IP, DP, IP]),
     "HIP_MatGetOrder": (c_int, [c_void_p, IP]),
+    "HIP_MatGatherKind": (c_int, [c_void_p]),
     "HIP_MatDestroy": (None, [C.POINTER(c_void_p)]),
     "HIP_MatNumRows": (c_int, [c_void_p]),
     "HIP_MatNumNonzeros": (c_int, [c_void_p]),

@@ -43,11 +43,14 @@ HIP_MAT *HIP_MatCreateFromCSR(GCGE_INT nrows, GCGE_INT ncols, GCGE_INT nnz, cons
  * endian, as HIP_WriteMatFile writes it. */
 HIP_MAT *HIP_ReadMatFile(const char *filename);
 /* A locality order of the rows of a square CSR pattern (host only, no
- * coordinates needed): perm[new position] = row, built by recursive halving
- * of the matrix graph in breadth-first order, parts of at most leaf rows
- * (0: 512) in breadth-first order.  On a grid it groups rows into compact
- * pieces, so a 256-row block reads its own rows plus about as many again
- * (its surface) instead of whole neighbouring planes.  Returns 0, or
+ * coordinates needed): perm[new position] = row.  A box grid numbered
+ * lexicographically is recognized from the pattern (at least 97 % of the
+ * entries within 2 grid steps in every direction) and listed in tiles of
+ * 8 x 8 x 8 (3-D) or 16 x 16 (2-D) rows; any other pattern is ordered by
+ * recursive halving of its graph in breadth-first order, parts of at most
+ * leaf rows (0: 512) in breadth-first order.  Either way rows come in
+ * compact pieces, so a 256-row block reads its own rows plus about as many
+ * again (its surface) instead of whole neighbouring planes.  Returns 0, or
  * non-zero for a bad pattern. */
 GCGE_INT HIP_MatLocalityOrder(GCGE_INT nrows, const GCGE_INT *rowptr, const GCGE_INT *col, GCGE_INT leaf,
                               GCGE_INT *perm);
@@ -64,6 +67,11 @@ HIP_MAT *HIP_MatCreateFromCSROrdered(GCGE_INT nrows, GCGE_INT ncols, GCGE_INT nn
 /* Writes the matrix's row order into perm[n] and returns 1, or returns 0
  * (natural order, perm untouched); -1 for a bad matrix. */
 GCGE_INT HIP_MatGetOrder(const HIP_MAT *mat, GCGE_INT *perm);
+/* The SpMM layout chosen at creation: 0 direct column gather (k_spmm),
+ * 1 row-major gather (k_spmm_rows), 2 staged row blocks (k_spmm_halo);
+ * -1 for a bad matrix.  No counterpart in the reference (a query of this
+ * backend's layout choice). */
+GCGE_INT HIP_MatGatherKind(const HIP_MAT *mat);
 /* Write CSR arrays (host) in the text format (binary = 0; values with 17
  * significant digits, so they read back exactly) or the binary one. */
 GCGE_INT HIP_WriteMatFile(const char *filename, GCGE_INT nrows, GCGE_INT ncols, GCGE_INT nnz,
@@ -150,7 +158,7 @@ typedef struct {
  * GCGE_HIP_Solver_Init with FP64 evec; its para (nev, ev_tol, conv_type, ...)
  * applies to the refinement.  When given_init_evec is set, evec holds the
  * start block.  opts may be NULL: fp32_tol 1e-7, the caller's ev_max_it for
- * both phases, stagnation window 10 / ratio 0.7 / start 1e-3, CholQR in both
+ * both phases, stagnation window 2 / ratio 0.5 / start 1e-3, CholQR in both
  * phases, FP32 inner solves in the refinement, inner solves preconditioned
  * with the diagonal of A and capped at 30 CG steps, and Rayleigh-Ritz pairs
  * from the full dsyevd decomposition.  In opts, a zero fp32_tol, budget or

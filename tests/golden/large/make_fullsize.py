@@ -1,6 +1,6 @@
-"""Writes tests/golden/large/{lap2d128,vardiff2d512}_ref.npz by running the
+"""Writes tests/golden/large/{lap2d128,vardiff2d512,lap3dmass96}_ref.npz by running the
 real reference (oracle/_ref/ref_solve, built from the reference's own sources
-with `make -C oracle ref`) at the full sizes of two benchmark workloads:
+with `make -C oracle ref`) at the full sizes of three benchmark workloads:
 
 * lap2d128:   2-D 5-point Dirichlet Laplacian on the 128 x 128 interior grid
               (problems.lap2d(128), n = 16,384), lowest 20 eigenpairs;
@@ -12,13 +12,16 @@ Both start from X0 = problems.random_x0(n, nev, seed) and use the
 reference's relative criterion "R" (||r|| / ||x|| / max(1, |lambda|)) with
 ev_tol = 1e-8 * 0.95 * lambda_1: with B = I and every wanted eigenvalue below
 1, R <= ev_tol implies ||A x - lambda x|| / (|lambda| ||x||) <= 1e-8 for every
-pair.  The generalized case (lap3dmass96) keeps the reference's default
-ev_tol 1e-8.  lambda_1 comes from scipy's shift-invert Lanczos before the run.  The
-fixture keeps the generator parameters, a fingerprint of the matrix and the
+pair.  For the generalized case (lap3dmass96: 3-D Laplacian with the P1
+mass matrix on 96^3, n = 884,736, lowest 50 pairs) every wanted eigenvalue
+is above 1 and ev_tol = 1e-8 * 0.95 * min ||Bx|| / ||x|| over the wanted
+modes (closed form), which again implies NS <= 1e-8.  lambda_1 comes from
+scipy's shift-invert Lanczos before the run.  The fixture keeps the generator parameters, a fingerprint of the matrix and the
 reference's outputs (eigenvalues, residuals, iteration count, per-iteration
 history); the test regenerates matrix and X0.
 
-Run (in a container that has /root/reference; a few minutes on one core):
+Run (in a container that has /root/reference; minutes on one core for the
+2-D cases, about an hour for lap3dmass96):
     MKL_NUM_THREADS=1 python tests/golden/large/make_fullsize.py [name ...]
 """
 from __future__ import annotations
@@ -41,10 +44,23 @@ CASES = {
     "lap2d128": dict(gen="lap2d", m=128, gen_seed=0, nev=20, x0_seed=5),
     "vardiff2d512": dict(gen="vardiff2d", m=512, gen_seed=0, nev=16, x0_seed=6),
     # the generalized workload: 3-D Laplacian with the P1 mass matrix on 96^3
-    # (n = 884,736), lowest 50 pairs, the reference's default ev_tol 1e-8 on
-    # its "R" criterion (about 35 minutes on one core)
-    "lap3dmass96": dict(gen="lap3d_mass", m=96, gen_seed=0, nev=50, x0_seed=7, ev_tol=1e-8),
+    # (n = 884,736), lowest 50 pairs
+    "lap3dmass96": dict(gen="lap3d_mass", m=96, gen_seed=0, nev=50, x0_seed=7),
 }
+
+
+def mass_floor(m, nev):
+    """Smallest ||Bx|| / ||x|| over the lowest nev modes of lap3d_mass(m):
+    the modes are products of sines, eigenvectors of B as well, with
+    B-eigenvalue mu_i mu_j mu_k, mu_i = (h/6)(4 + 2 cos(i pi h))."""
+    th = np.arange(1, m + 1) * np.pi / (m + 1)
+    t = 2.0 - 2.0 * np.cos(th)
+    mu = (1.0 / (m + 1) / 6.0) * (4.0 + 2.0 * np.cos(th))
+    c = min(m, int(np.ceil(nev ** (1.0 / 3.0))) + 6)
+    lam = (t[:c, None, None] + t[None, :c, None] + t[None, None, :c]) / (
+        mu[:c, None, None] * mu[None, :c, None] * mu[None, None, :c])
+    prod = mu[:c, None, None] * mu[None, :c, None] * mu[None, None, :c]
+    return float(prod.ravel()[np.argsort(lam.ravel(), kind="stable")[:nev]].min())
 
 
 def matrix(case):
@@ -68,8 +84,11 @@ def main(names):
         case = CASES[name]
         A, B = matrix(case)
         n, nev = A.shape[0], case["nev"]
-        if "ev_tol" in case:
-            lam1, tol = float("nan"), case["ev_tol"]
+        if B is not None:
+            # every wanted eigenvalue is above 1, so "R" is ||r|| / (lambda ||x||)
+            # and NS = R ||x|| / ||Bx|| <= R / min ||Bx|| / ||x||
+            lam1 = float("nan")
+            tol = float("%.3e" % (1e-8 * 0.95 * mass_floor(case["m"], nev)))
         else:
             lam1 = float(sla.eigsh(A.tocsc(), k=1, sigma=0.0, which="LM", return_eigenvectors=False)[0])
             tol = float("%.3e" % (1e-8 * 0.95 * lam1))

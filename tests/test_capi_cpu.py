@@ -259,6 +259,60 @@ def test_finite_residuals_unchanged_rule():
     L.GCGE_PARA_Free(C.byref(p))
 
 
+@pytest.mark.parametrize("nev,bs,max_dim_x", [
+    (1, 1, 4),      # nev * 1.25 rounds to 1; the nev + 3 floor applies
+    (10, 5, 13),    # 12 = int(12.5) < nev + 8, below the nev + 3 floor
+    (20, 20, 25),   # int(25.0) < nev + 8 = 28
+    (40, 40, 48),   # nev + 8 = 48 < 50
+    (100, 100, 108),
+    (100, 36, 108),
+])
+def test_workspace_sizes_follow_reference_rule(nev, bs, max_dim_x):
+    """GCGE_WORKSPACE_Setup requests the multivectors gcge_workspace.c:61-70
+    sizes: max_dim_x = max(min(int(1.25 nev), nev + 8), nev + 3), V of
+    max_dim_x + 2 bs columns, V_tmp of max_dim_x - nev + bs, CG_p of bs.  The
+    expected max_dim_x values are worked out by hand for each branch of that
+    rule; the ops table records the requests (no device involved)."""
+    L = capi.load()
+    p = C.POINTER(capi.GCGE_PARA)()
+    L.GCGE_PARA_Create(C.byref(p))
+    L.GCGE_PARA_SetNumEigen(p, nev)
+    p.contents.block_size = bs
+    L.GCGE_PARA_Setup(p)
+    assert p.contents.nev == nev and p.contents.block_size == bs
+    ops = capi.GCGE_OPS()
+    requests, destroyed = [], []
+    handles = [C.c_int(0) for _ in range(3)]
+
+    F_create = C.CFUNCTYPE(None, C.POINTER(C.c_void_p), C.c_int, C.c_void_p, C.c_void_p)
+    F_destroy = C.CFUNCTYPE(None, C.POINTER(C.c_void_p), C.c_int, C.c_void_p)
+
+    def create(out, k, A, o):
+        out[0] = C.cast(C.byref(handles[len(requests)]), C.c_void_p).value
+        requests.append(k)
+
+    def destroy(h, k, o):
+        destroyed.append(k)
+        h[0] = None
+
+    cbc, cbd = F_create(create), F_destroy(destroy)
+    ops.MultiVecCreateByMat = C.cast(cbc, C.c_void_p)
+    ops.MultiVecDestroy = C.cast(cbd, C.c_void_p)
+    ws = C.POINTER(capi.GCGE_WORKSPACE)()
+    L.GCGE_WORKSPACE_Create(C.byref(ws))
+    L.GCGE_WORKSPACE_Setup(ws, p, C.byref(ops), None)
+    W = ws.contents
+    assert requests == [max_dim_x + 2 * bs, max_dim_x - nev + bs, bs]
+    assert (W.V_size, W.V_tmp_size, W.CG_p_size) == tuple(requests)
+    assert W.max_dim_x == max_dim_x
+    assert (W.dim_x, W.dim_xp, W.dim_xpw, W.conv_bs) == (nev, 0, nev, 0)
+    assert W.unconv_bs == min(nev, bs)
+    assert [W.unlock[i] for i in range(max_dim_x)] == list(range(max_dim_x))
+    L.GCGE_WORKSPACE_Free(C.byref(ws), p, C.byref(ops))
+    assert destroyed == requests
+    L.GCGE_PARA_Free(C.byref(p))
+
+
 def test_w_orth_types_parse():
     """-gcge_w_orth_type accepts the reference's names (multi, gs, bgs, cbgs,
     scbgs) and this build's cholqr."""
@@ -329,6 +383,36 @@ def _block_extras(M, perm, R=256):
     return np.array(out)
 
 
+@pytest.mark.parametrize("dims,tile", [((13, 11, 9), (8, 8, 8)), ((40, 24, 1), (16, 16, 1)),
+                                       ((17, 16, 16), (8, 8, 8))])
+def test_locality_order_tiles_a_box_grid(dims, tile):
+    """A lexicographically numbered box grid (7-point in 3-D, 5-point in
+    2-D; extents not multiples of the tile) comes back tile by tile: the
+    expected order is built here from grid coordinates, tiles clipped at
+    the faces, tiles and rows within a tile x fastest."""
+    import scipy.sparse as sp
+    from gcge_hip import locality_order
+    nx, ny, nz = dims
+    tx, ty, tz = tile
+
+    def lap1(m):
+        return sp.diags([-np.ones(m - 1), 2 * np.ones(m), -np.ones(m - 1)], [-1, 0, 1])
+
+    I = lambda m: sp.identity(m)
+    if nz > 1:
+        A = sp.kron(sp.kron(I(nz), I(ny)), lap1(nx)) + sp.kron(sp.kron(I(nz), lap1(ny)), I(nx)) + \
+            sp.kron(sp.kron(lap1(nz), I(ny)), I(nx))
+    else:
+        A = sp.kron(I(ny), lap1(nx)) + sp.kron(lap1(ny), I(nx))
+    A = A.tocsr()
+    A.sort_indices()
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    row = ((z * ny + y) * nx + x).ravel()
+    key = np.lexsort(((x % tx).ravel(), (y % ty).ravel(), (z % tz).ravel(),
+                      (x // tx).ravel(), (y // ty).ravel(), (z // tz).ravel()))
+    np.testing.assert_array_equal(locality_order(A), row[key])
+
+
 def test_locality_order_is_a_permutation_with_compact_blocks():
     """HIP_MatLocalityOrder (host only): a permutation of the rows; on a 3-D
     7-point grid its 256-row blocks read about their own size in other rows
@@ -347,6 +431,14 @@ def test_locality_order_is_a_permutation_with_compact_blocks():
     G, _ = P.chung_lu(5000, mean_deg=2.0, seed=3)
     pg = locality_order(G, leaf=64)
     assert np.array_equal(np.sort(pg), np.arange(G.shape[0]))
+    # a grid graph whose numbering is scrambled is no box in natural order:
+    # the recursive halving still finds compact blocks
+    rng = np.random.default_rng(0)
+    q = rng.permutation(n)
+    S = A[q][:, q].tocsr()
+    ps = locality_order(S)
+    assert np.array_equal(np.sort(ps), np.arange(n))
+    assert _block_extras(S, ps).mean() < 0.6 * nat
     with pytest.raises(ValueError):
         bad = A.copy()
         bad.indices[0] = n + 5

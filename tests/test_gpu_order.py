@@ -44,8 +44,12 @@ def test_ordered_spmm_and_transfers_in_callers_order():
     mB.close()
 
 
+@pytest.mark.parametrize("order", ["locality", "random"])
 @pytest.mark.parametrize("name", ["lap2d32_x0", "lap3dmass8_x0", "vardiff2d16_x0"])
-def test_ordered_solve_matches_reference_fixture(name):
+def test_ordered_solve_matches_reference_fixture(name, order):
+    """The reference's fixtures solved on P A P^T: the locality order (16 x 16
+    tiles of the 32 x 32 grid; the identity for grids of one tile) and a
+    random permutation of the rows."""
     import ast
     import scipy.sparse as sp
     d = np.load(os.path.join(GOLD, name + ".npz"))
@@ -53,7 +57,7 @@ def test_ordered_solve_matches_reference_fixture(name):
     A = sp.csr_matrix((d["A_data"], d["A_indices"], d["A_indptr"]), shape=(n, n))
     B = sp.csr_matrix((d["B_data"], d["B_indices"], d["B_indptr"]), shape=(n, n)) if bool(d["has_B"]) else None
     kw = ast.literal_eval(str(d["orc_kwargs"]))
-    mA = Matrix(A, order="locality")
+    mA = Matrix(A, order="locality" if order == "locality" else np.random.default_rng(3).permutation(n))
     mB = Matrix(B, order=mA.order()) if B is not None else None
     r = solve(mA, mB, int(d["nev"]), X0=d["X0"], **kw)
     np.testing.assert_allclose(r["eval"], d["eval"], rtol=1e-10)
