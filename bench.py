@@ -299,6 +299,9 @@ def main():
     ap.add_argument("--stagnation-start", type=float, default=1e-3)
     ap.add_argument("--order", default="auto", choices=["auto", "locality", "natural"],
                     help="row order of the device matrices (auto: locality for the grid workloads)")
+    ap.add_argument("--leg", default="all", choices=["all", "mixed", "fp64", "fp64-same"],
+                    help="run one leg alone (mixed; FP64 with the reference's defaults; FP64 with the mixed "
+                         "path's settings), for profiler runs; no JSON bench line")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 solve (reference defaults)")
     ap.add_argument("--cg-max-it", type=int, default=30,
                     help="CG step cap of the mixed path's inner solves (the FP64 drop-in keeps the reference's 20)")
@@ -390,6 +393,21 @@ def main():
         kms = sum(v["ms"] for v in stats.values())
         log(f"{what} (profiled): {dt:.3f}s, kernels {kms / 1e3:.3f}s")
         return r, dt, stats, kms
+
+    if args.leg != "all":
+        # one leg alone, for profiler runs (rocprofv3 --kernel-trace / --pmc):
+        # the solver of that leg, args.warmup + args.steps solves from X0
+        kw = dict(precision="mixed", mixed=mixed) if args.leg == "mixed" else dict(precision="fp64")
+        if args.leg == "fp64-same":
+            kw.update(w_orth_type="cholqr", cg_max_it=args.cg_max_it, bcg_precond=args.bcg_precond, rr_eig_full=1)
+        S = Solver(mA, mB, nev, **kw, **common)
+        out = []
+        for i in range(args.warmup + args.steps):
+            r, dt = timed(S, f"{args.leg} solve {i}")
+            out.append(dict(seconds=round(dt, 4), iters=r["num_iter"], converged=bool(ns_of(S, r)[1])))
+        S.close()
+        print(json.dumps({"leg": args.leg, "workload": f"{args.workload} {args.size}", "nev": nev, "solves": out}))
+        return
 
     # ---- the timed steps: one solver set up once (workspace allocated
     # outside the timed region), solves from the same X0, profiler off
