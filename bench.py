@@ -103,7 +103,8 @@ ROCPROF_KERNEL = {
 
 def load_traffic(run, workload):
     """HBM bytes per launch measured with rocprofv3 --pmc for the kernel
-    classes of one run ("mixed" or "fp64") of the workload they were
+    classes of one run ("mixed", or "fp64_same": FP64 with the mixed
+    path's settings) of the workload they were
     profiled on (profiles/traffic.json); nothing for another workload."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
@@ -525,7 +526,7 @@ def main():
                                           w_orth_type="cholqr", cg_max_it=args.cg_max_it,
                                           bcg_precond=args.bcg_precond, rr_eig="dsyevd")
         if "stats" in fp64s:
-            d64, roof64 = roofline(fp64s["stats"], "fp64", wl)
+            d64, roof64 = roofline(fp64s["stats"], "fp64_same", wl)
             line["fp64_same_settings"].update(
                 roofline_kernel=d64, roofline=roof64,
                 kernels={k: dict(count=v["count"], ms=round(v["ms"], 2),
