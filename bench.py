@@ -147,7 +147,10 @@ def cpu_baseline(args, n_full, nnz_full, fp64_iters, nev):
     full problem and multiplied by the GPU FP64 iteration count."""
     import oracle_api as O
     from gcge_hip import problems as P
-    m = args.cpu_sample_size
+    # sample sizes: a few seconds of reference work per iteration, beyond the
+    # host's caches (the 2-D Laplacian is small enough to run at full size)
+    m = args.cpu_sample_size or {"vardiff3d": 48, "lap3d_mass": 48, "lap2d": args.size, "vardiff2d": 256,
+                                 "chung_lu": 20000}.get(args.workload, 48)
     its = args.cpu_sample_iters
     if args.workload == "vardiff3d":
         A, B = P.vardiff3d(m, seed=args.seed)
@@ -315,7 +318,9 @@ def main():
     ap.add_argument("--no-profiled-step", action="store_true",
                     help="skip the extra profiled solve that gives the per-kernel rates")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-sample-size", type=int, default=48)
+    ap.add_argument("--cpu-sample-size", type=int, default=0,
+                    help="generator size of the CPU sample (0: 48 for the 3-D grids, 256 for vardiff2d, "
+                         "the full size for lap2d, 20000 vertices for chung_lu)")
     ap.add_argument("--cpu-sample-iters", type=int, default=3)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run k_copy<double, double> over 16 columns of n = 4,096,000 rows three times "
