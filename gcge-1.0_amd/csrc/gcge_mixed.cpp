@@ -89,7 +89,12 @@ extern "C" GCGE_INT GCGE_HIP_SOLVER_SolveMixed(GCGE_SOLVER *solver, const GCGE_H
     GCGE_PARA *P = solver->para;
     const int nev = P->nev;
     GCGE_HIP_MIXED_OPTS o;
-    o.fp32_tol = (opts && opts->fp32_tol > 0) ? opts->fp32_tol : 1e-7;
+    // the FP32 phase locks each pair at a residual of 8e-3, above the FP32
+    // rounding floor of the benchmark problems (about 3e-3 on the 160^3
+    // diffusion operator): run to the floor instead, every pair keeps
+    // iterating on rounding noise and the refinement needed 47 iterations
+    // there against 39 (all five benchmark workloads were as fast or faster)
+    o.fp32_tol = (opts && opts->fp32_tol > 0) ? opts->fp32_tol : 8e-3;
     o.fp32_max_it = (opts && opts->fp32_max_it > 0) ? opts->fp32_max_it : P->ev_max_it;
     o.fp64_max_it = (opts && opts->fp64_max_it > 0) ? opts->fp64_max_it : P->ev_max_it;
     // without opts: the configuration of the mixed path (FP32 phase with CholQR,

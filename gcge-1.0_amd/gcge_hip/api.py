@@ -331,7 +331,7 @@ def synchronize():
 
 # GCGE_HIP_MIXED_OPTS of the two-phase forms (0 selects the library's default for
 # fp32_tol, the iteration budgets -- the caller's ev_max_it -- and stagnation_ratio)
-MIXED_DEFAULTS = dict(fp32_tol=1e-7, fp32_max_it=0, fp64_max_it=0, stagnation_window=2, stagnation_ratio=0.5,
+MIXED_DEFAULTS = dict(fp32_tol=8e-3, fp32_max_it=0, fp64_max_it=0, stagnation_window=2, stagnation_ratio=0.5,
                       fp32_w_orth=1, refine_w_orth=1, refine_bcg_precision=1, stagnation_start=1e-3, cg_max_it=30,
                       bcg_precond=1, rr_eig_full=1)
 FP32_REFINE_DEFAULTS = dict(MIXED_DEFAULTS, fp32_w_orth=0, refine_w_orth=0, refine_bcg_precision=0, cg_max_it=0,

@@ -157,7 +157,7 @@ typedef struct {
  * eigenvectors (given_init_evec).  solver must have been created by
  * GCGE_HIP_Solver_Init with FP64 evec; its para (nev, ev_tol, conv_type, ...)
  * applies to the refinement.  When given_init_evec is set, evec holds the
- * start block.  opts may be NULL: fp32_tol 1e-7, the caller's ev_max_it for
+ * start block.  opts may be NULL: fp32_tol 8e-3, the caller's ev_max_it for
  * both phases, stagnation window 2 / ratio 0.5 / start 1e-3, CholQR in both
  * phases, FP32 inner solves in the refinement, inner solves preconditioned
  * with the diagonal of A and capped at 30 CG steps, and Rayleigh-Ritz pairs
