@@ -303,6 +303,8 @@ def main():
     ap.add_argument("--stagnation-start", type=float, default=1e-3)
     ap.add_argument("--order", default="auto", choices=["auto", "locality", "natural"],
                     help="row order of the device matrices (auto: locality for the grid workloads)")
+    ap.add_argument("--rr-eig-partly", type=int, default=1, choices=[0, 1],
+                    help="soft locking (opt_rr_eig_partly) in the mixed and same-settings FP64 solves")
     ap.add_argument("--leg", default="all", choices=["all", "mixed", "fp64", "fp64-same"],
                     help="run one leg alone (mixed; FP64 with the reference's defaults; FP64 with the mixed "
                          "path's settings), for profiler runs; no JSON bench line")
@@ -361,6 +363,9 @@ def main():
     log(f"upload of the matrix and X0 over PCIe: {time.perf_counter() - t:.2f}s")
 
     common = dict(conv_type="NS", ev_tol=args.tol, ev_max_it=args.max_it, print_result=1 if args.verbose else 0)
+    # settings of the mixed path that the same-settings FP64 solve shares
+    # (the FP64 drop-in keeps the reference's defaults)
+    tuned = dict(opt_rr_eig_partly=args.rr_eig_partly)
     mixed = dict(fp32_tol=args.fp32_tol, fp32_max_it=args.max_it, fp64_max_it=args.max_it,
                  stagnation_window=args.stagnation_window, stagnation_ratio=args.stagnation_ratio,
                  stagnation_start=args.stagnation_start, cg_max_it=args.cg_max_it, bcg_precond=args.bcg_precond)
@@ -403,9 +408,10 @@ def main():
     if args.leg != "all":
         # one leg alone, for profiler runs (rocprofv3 --kernel-trace / --pmc):
         # the solver of that leg, args.warmup + args.steps solves from X0
-        kw = dict(precision="mixed", mixed=mixed) if args.leg == "mixed" else dict(precision="fp64")
+        kw = dict(precision="mixed", mixed=mixed, **tuned) if args.leg == "mixed" else dict(precision="fp64")
         if args.leg == "fp64-same":
-            kw.update(w_orth_type="cholqr", cg_max_it=args.cg_max_it, bcg_precond=args.bcg_precond, rr_eig_full=1)
+            kw.update(w_orth_type="cholqr", cg_max_it=args.cg_max_it, bcg_precond=args.bcg_precond, rr_eig_full=1,
+                      **tuned)
         S = Solver(mA, mB, nev, **kw, **common)
         out = []
         for i in range(args.warmup + args.steps):
@@ -417,7 +423,7 @@ def main():
 
     # ---- the timed steps: one solver set up once (workspace allocated
     # outside the timed region), solves from the same X0, profiler off
-    S = Solver(mA, mB, nev, precision=args.mode, mixed=mixed, **common)
+    S = Solver(mA, mB, nev, precision=args.mode, mixed=mixed, **(tuned if args.mode != "fp64" else {}), **common)
     warm = []
     for i in range(args.warmup):
         warm.append(timed(S, f"warmup {i}")[1])
@@ -456,7 +462,7 @@ def main():
     if not args.no_fp64_same and args.mode == "mixed":
         # FP64 with every algorithm setting of the mixed run: timed like a step
         S64s = Solver(mA, mB, nev, precision="fp64", w_orth_type="cholqr", cg_max_it=args.cg_max_it,
-                      bcg_precond=args.bcg_precond, rr_eig_full=1, **common)
+                      bcg_precond=args.bcg_precond, rr_eig_full=1, **tuned, **common)
         r64s, t64s = timed(S64s, "fp64 with the mixed path's settings")
         ns64s, ok64s = ns_of(S64s, r64s)
         fp64s = dict(seconds=t64s, iters=r64s["num_iter"], converged=ok64s, eval=np.asarray(r64s["eval"]),
@@ -497,6 +503,7 @@ def main():
                    "mode": args.mode, "fp32_tol": args.fp32_tol, "cg_max_it": args.cg_max_it,
                    "bcg_precond": "jacobi" if args.bcg_precond else "none",
                    "rr_eig": "dsyevd" if args.mode == "mixed" else "dsyevx",
+                   "soft_locking": bool(args.rr_eig_partly) if args.mode != "fp64" else False,
                    "stagnation": [args.stagnation_window, args.stagnation_ratio, args.stagnation_start],
                    "row_order": "locality (HIP_MatLocalityOrder)" if order else "natural",
                    "spmm": spmm_kind(mA)},
@@ -524,12 +531,13 @@ def main():
     if fp64s:
         # the precision-only comparison: FP64 with every algorithm setting of
         # the mixed path (CholQR, inner CG step cap and preconditioner, dense
-        # Rayleigh-Ritz solver), timed the same way as the steps
+        # Rayleigh-Ritz solver, soft locking), timed the same way as the steps
         line["fp64_same_settings"] = dict(eigenpairs_per_s=round(nev / fp64s["seconds"], 4),
                                           seconds=round(fp64s["seconds"], 3), iters=fp64s["iters"],
                                           converged=fp64s["converged"], max_ns_residual=fp64s["max_ns"],
                                           w_orth_type="cholqr", cg_max_it=args.cg_max_it,
-                                          bcg_precond=args.bcg_precond, rr_eig="dsyevd")
+                                          bcg_precond=args.bcg_precond, rr_eig="dsyevd",
+                                          soft_locking=bool(args.rr_eig_partly))
         if "stats" in fp64s:
             d64, roof64 = roofline(fp64s["stats"], "fp64_same", wl)
             line["fp64_same_settings"].update(

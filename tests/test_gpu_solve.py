@@ -182,6 +182,23 @@ def test_random_init_uses_reference_sequence():
 MIXED_FORMS = ["fp64-cg32", "mixed", "fp32+refine"]
 
 
+@pytest.mark.parametrize("partly", [0, 1])
+def test_mixed_soft_locking_reaches_fp64_answer(partly):
+    """The mixed path with soft locking (opt_rr_eig_partly, the reference's
+    option: only the unconverged Ritz pairs are recomputed) and with its
+    default settings ends on the FP64 answer: eigenvalues within 1e-10 of the
+    FP64 solve, every pair at the tolerance."""
+    A, _ = P.vardiff3d(24, seed=3)
+    nev = 16
+    X0 = P.random_x0(A.shape[0], nev, 4)
+    r64 = solve(A, None, nev, X0=X0, conv_type="NS", ev_tol=1e-8, ev_max_it=300)
+    rmx = solve(Matrix(A, order="locality"), None, nev, X0=X0, conv_type="NS", ev_tol=1e-8, ev_max_it=300,
+                precision="mixed", opt_rr_eig_partly=partly)
+    assert rmx["num_unlock"] == 0 and rmx["rc"] == 0
+    np.testing.assert_allclose(rmx["eval"], r64["eval"], rtol=EV_RTOL)
+    assert np.all(ns_residual(A, None, rmx["evec"], rmx["eval"]) <= 1e-8 * (1 + 1e-6))
+
+
 @pytest.mark.parametrize("precision", MIXED_FORMS)
 def test_mixed_precision_reaches_fp64_answer(precision):
     """Every mixed form ends on the FP64 answer: eigenvalues within 1e-10 of
