@@ -165,8 +165,11 @@ void orth_gs(void **V, int start, int *end, void *B, GCGE_OPS *ops, GCGE_PARA *p
 }
 
 // GCGE_SubOrthonormalization, gcge_orthogonal.c:1880-1929:
-// V2 = V2 - V1 (V1^T B V2), as one Gram and one fused combination.
-void sub_orth(void **V, const GCGE_INT *start, const GCGE_INT *end, void *B, void **V_tmp, double *d, GCGE_OPS *ops)
+// V2 = V2 - V1 (V1^T B V2), as one Gram and one fused combination.  With
+// wn2 (not null), the squared B-norms of the V2 columns before the
+// projection are written there as well.
+void sub_orth(void **V, const GCGE_INT *start, const GCGE_INT *end, void *B, void **V_tmp, double *d, GCGE_OPS *ops,
+              double *wn2 = nullptr)
 {
     const int len1 = end[0] - start[0], len2 = end[1] - start[1];
     GCGE_INT ms[2], me[2];
@@ -182,6 +185,11 @@ void sub_orth(void **V, const GCGE_INT *start, const GCGE_INT *end, void *B, voi
     set2(ms, start[0], tc);
     set2(me, end[0], tc + len2);
     ops->MultiVecInnerProd(V, T, d, (char *)"ns", ms, me, len1, ops);
+    if (wn2) {
+        set2(ms, start[1], tc);
+        set2(me, end[1], tc + len2);
+        ops->MultiVecInnerProdDiag(V, T, wn2, ms, me, ops);
+    }
     set2(ms, start[0], start[1]);
     set2(me, end[0], end[1]);
     ops->MultiVecLinearCombAxpby(V, V, ms, me, d, len1, -1.0, 1.0, ops);
@@ -648,13 +656,35 @@ void cholqr_orth(void **V, int start, int *end, void *B, GCGE_OPS *ops, GCGE_PAR
 {
     GCGE_INT ms[2], me[2];
     const double check_tol = 1e3 * para->basis_eps;
-    std::vector<double> G;
+    std::vector<double> G, wn2;
+    // W from the FP32 defect correction is the correction E alone (compute_w),
+    // mostly orthogonal to V1 already: when the first projection removes at
+    // most half of every column's squared B-norm, the second is skipped
+    // ("twice is enough": one pass then loses no more than a factor 2 of
+    // orthogonality to cancellation, and the factor pass and the check below
+    // still apply)
+    const bool correction = para->bcg_precision == 1 && para->if_use_bcg == 1 && !ops->LinearSolver;
     for (int round = 0; round < 3 && *end > start; round++) {
         if (start > 0) {
-            for (int pass = 0; pass < 2 && *end > start; pass++) {
+            bool again = true;
+            for (int pass = 0; pass < 2 && *end > start && again; pass++) {
                 set2(ms, 0, start);
                 set2(me, start, *end);
-                sub_orth(V, ms, me, B, ws->V_tmp, ws->subspace_dtmp, ops);
+                const int len2 = *end - start;
+                const bool test = correction && pass == 0;
+                if (test) wn2.assign(len2, 0.0);
+                sub_orth(V, ms, me, B, ws->V_tmp, ws->subspace_dtmp, ops, test ? wn2.data() : nullptr);
+                if (test) {
+                    again = false;
+                    for (int j = 0; j < len2 && !again; j++) {
+                        double g2 = 0.0;
+                        for (int i = 0; i < start; i++) {
+                            const double g = ws->subspace_dtmp[(size_t)j * start + i];
+                            g2 += g * g;
+                        }
+                        if (!(g2 <= 0.5 * wn2[j])) again = true;
+                    }
+                }
             }
         }
         double rmin = 1.0;
@@ -1078,8 +1108,9 @@ void compute_w(void *A, void *B, void **V, double *eval, GCGE_OPS *ops, GCGE_PAR
         // Defect correction: r0 = rhs - A W0 in FP64 (the first step of the
         // reference's BCG), CG for the correction A E = r0 from E = 0 on FP32
         // copies (the same CG iterates in exact arithmetic, and the same
-        // ||r_k|| / ||r_0|| stopping rule), then W = W0 + E in FP64.  FP32
-        // rounding is relative to the correction, which shrinks as X converges.
+        // ||r_k|| / ||r_0|| stopping rule); W0 + E and E span the same space
+        // with X.  FP32 rounding is relative to the correction, which shrinks
+        // as X converges.
         if (!bcg32_ready(A, w_length, ops, ws) || !pcg_ready(ops, para)) return;
         set2(ms, w_start, 0);
         set2(me, w_start + w_length, w_length);
@@ -1091,10 +1122,12 @@ void compute_w(void *A, void *B, void **V, double *eval, GCGE_OPS *ops, GCGE_PAR
         ops->MultiVecAxpby(0.0, ws->bcg32[1], 0.0, ws->bcg32[1], ms, me, ops);
         bcg(A, ws->bcg32[0], ws->bcg32[1], 0, w_length, ops, para, ws->bcg32[2], ws->bcg32[3], ws->subspace_dtmp,
             ws->subspace_itmp, true);
-        ops->MultiVecCopy(ws->bcg32[1], ws->V_tmp, ms, me, ops);
+        // W = W0 + E spans the same space with X as E alone (W0 is a column of X
+        // scaled), so E, widened to FP64, is the W block: it is mostly
+        // orthogonal to X already, which the W orthonormalization uses
         set2(ms, 0, w_start);
         set2(me, w_length, w_start + w_length);
-        ops->MultiVecAxpby(1.0, ws->V_tmp, 1.0, V, ms, me, ops);
+        ops->MultiVecCopy(ws->bcg32[1], V, ms, me, ops);
     } else if (para->if_use_bcg == 1) {
         if (!pcg_ready(ops, para)) return;
         bcg(A, ws->V_tmp, V, w_start, w_length, ops, para, ws->CG_p, ws->evec, ws->subspace_dtmp, ws->subspace_itmp);

@@ -131,8 +131,9 @@ typedef struct GCGE_PARA_ {
      * (0: same as the basis; 1: FP32 defect correction -- the initial
      * residual r0 = b - A W0 is formed in FP64, the CG for the correction
      * A E = r0 runs on FP32 vectors and FP32 matrix values with
-     * FP64-accumulated dots, and W = W0 + E is formed in FP64).  Needs the
-     * MultiVecCreateByMatPrecision slot. */
+     * FP64-accumulated dots, and the W block is E, widened to FP64: W0 is a
+     * scaled column of X, so W0 + E and E span the same space with X).
+     * Needs the MultiVecCreateByMatPrecision slot. */
     GCGE_INT bcg_precision;
     /* input: when > 0, stagnation is only watched once max_res has fallen to
      * this fraction of its largest value so far (residuals of a random start
