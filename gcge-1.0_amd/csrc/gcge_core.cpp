@@ -1414,6 +1414,17 @@ extern "C" void GCGE_Solve(void *A, void *B, GCGE_DOUBLE *eval, void **evec, GCG
         return;
     }
 
+    // With w_orth_type "cholqr" the initial block is B-orthonormalized the same
+    // way (one block factorization instead of the column-by-column
+    // Gram-Schmidt of the reference, about 0.09 s against a few ms for 100
+    // columns of 4.1M rows), when V_tmp can hold the block.
+    const bool chol_init = std::strcmp(para->w_orth_type, "cholqr") == 0 && ws->V_tmp_size >= nev;
+    auto orth_init = [&](int start, int *end) {
+        if (chol_init)
+            cholqr_orth(evec, start, end, Orth_mat, ops, para, ws);
+        else
+            orth_gs(evec, start, end, Orth_mat, ops, para, ws->V_tmp, ws->subspace_dtmp);
+    };
     {
         GCGE_TIMER(x_orth_time, para);
         if (para->given_init_evec == 0) {
@@ -1442,11 +1453,11 @@ extern "C" void GCGE_Solve(void *A, void *B, GCGE_DOUBLE *eval, void **evec, GCG
                 if (para->dirichlet_boundary == 1 && ops->SetDirichletBoundary)
                     ops->SetDirichletBoundary(evec, nev, A, B);
                 ws->dim_x = nev;
-                orth_gs(evec, num_init_evec, &ws->dim_x, Orth_mat, ops, para, ws->V_tmp, ws->subspace_dtmp);
+                orth_init(num_init_evec, &ws->dim_x);
                 num_init_evec = ws->dim_x;
             } while (num_init_evec < nev && !gh::failed());
         } else {
-            orth_gs(evec, 0, &ws->dim_x, Orth_mat, ops, para, ws->V_tmp, ws->subspace_dtmp);
+            orth_init(0, &ws->dim_x);
         }
     }
     GCGE_PrintParaInfo(para);
