@@ -1828,6 +1828,8 @@ extern "C" void GCGE_SOLVER_Solve(GCGE_SOLVER *s)
     P->num_unlock = nev;
     P->sum_res = 0.0;
     for (int i = 0; i < nev; i++) P->res[i] = 0.0;
+    // part times describe this solve (a reused solver starts them at zero)
+    if (P->stat_para && P->stat_para->part_time_total) *P->stat_para->part_time_total = GCGE_PART_TIME{};
     ws->dim_x = nev;
     ws->dim_xp = 0;
     ws->dim_xpw = nev;

@@ -211,6 +211,8 @@ extern "C" GCGE_INT GCGE_HIP_SOLVER_SolveMixed(GCGE_SOLVER *solver, const GCGE_H
     int iters = 0, bcg_iters = 0, bcg_calls = 0, failing = nev;
     bool nonfinite = false;
     std::vector<double> res(nev);
+    GCGE_PART_TIME parts{};  // each solve starts its part times at zero: add the passes up
+    GCGE_PART_TIME *pt = P->stat_para ? P->stat_para->part_time_total : nullptr;
     for (int pass = 0; pass < 3; pass++) {
         GCGE_SOLVER_Solve(solver);
         GCGE_HIP_Synchronize();
@@ -218,6 +220,17 @@ extern "C" GCGE_INT GCGE_HIP_SOLVER_SolveMixed(GCGE_SOLVER *solver, const GCGE_H
         iters += P->num_iter;
         bcg_iters += P->bcg_iters_total;
         bcg_calls += P->bcg_calls;
+        if (pt) {
+            parts.x_axpy_time += pt->x_axpy_time;
+            parts.x_orth_time += pt->x_orth_time;
+            parts.p_axpy_time += pt->p_axpy_time;
+            parts.p_orth_time += pt->p_orth_time;
+            parts.w_line_time += pt->w_line_time;
+            parts.w_orth_time += pt->w_orth_time;
+            parts.rr_mat_time += pt->rr_mat_time;
+            parts.rr_eigen_time += pt->rr_eigen_time;
+            parts.conv_time += pt->conv_time;
+        }
         if (GCGE_HIP_GetLastError(nullptr, 0) != 0) break;
         // every pair under the caller's criterion (locked pairs may have drifted)
         pair_residuals(solver, res.data());
@@ -230,6 +243,7 @@ extern "C" GCGE_INT GCGE_HIP_SOLVER_SolveMixed(GCGE_SOLVER *solver, const GCGE_H
         if (failing == 0 || nonfinite || P->num_unlock != 0) break;  // done, broken, or budget spent
         P->given_init_evec = 1;  // restart from the current vectors
     }
+    if (pt) *pt = parts;
     P->num_iter = iters;  // the statistics of the passes added up
     P->bcg_iters_total = bcg_iters;
     P->bcg_calls = bcg_calls;

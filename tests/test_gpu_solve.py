@@ -520,8 +520,9 @@ def test_set_values_refreshes_jacobi_diagonal():
 
 def test_solver_reuse_gives_identical_results():
     """A set-up solver solves again from a new initial block with the same
-    result as a fresh solver (workspace reused, per-solve state restarted),
-    also for the mixed path, whose FP32-phase solver is kept."""
+    result as a fresh solver (workspace reused, per-solve state restarted,
+    part times included), also for the mixed path, whose FP32-phase solver
+    is kept."""
     from gcge_hip import Solver
     A, _ = P.vardiff2d(32, seed=5)
     nev = 8
@@ -537,6 +538,9 @@ def test_solver_reuse_gives_identical_results():
         for x, y in ((ra, fa), (rb, fb), (ra2, fa)):
             assert x["num_iter"] == y["num_iter"] and x["num_unlock"] == 0
             np.testing.assert_array_equal(x["eval"], y["eval"])
+        # the third solve's part times cover that solve only, not all three
+        t1, t3 = sum(ra["part_time"].values()), sum(ra2["part_time"].values())
+        assert 0 < t3 < 2.0 * t1, (t1, t3)
 
 
 def test_mixed_power_law_graph_matches_reference():
