@@ -2,7 +2,7 @@
 //
 // One process drives one GPU through one HIP stream (default priority).
 // Multivectors are single column-major allocations whose leading dimension
-// is padded to a multiple of 256 elements, so every column starts on a
+// is padded to an odd multiple of 256 elements, so every column starts on a
 // 2 KiB (FP64) / 1 KiB (FP32) boundary and any column range can be streamed
 // with aligned, coalesced loads.
 #pragma once

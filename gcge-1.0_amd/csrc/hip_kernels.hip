@@ -431,20 +431,38 @@ __global__ __launch_bounds__(kSpmmRows) void k_spmm_halo(int n, const int *__res
     using SV = SlotVec<T, KC>;
     SV *sv = reinterpret_cast<SV *>(s_x);
     __syncthreads();
-    for (int c0 = 0; c0 < k; c0 += KC) {
-        const T *xc[KC];
+    // the own row and the first extra slot of this thread for the next KC
+    // columns are loaded into registers while the current columns are
+    // multiplied, so that a workgroup's loads overlap its own work
+    const int row1 = tid < nh ? s_h[tid] : 0;
+    SV pown, pext;
+    auto xcols = [&](int c0, const T *(&xc)[KC]) {
 #pragma unroll
         for (int cc = 0; cc < KC; cc++) {
             const int c = min(c0 + cc, k - 1);
             xc[cc] = X + (g.use_idx ? (int64_t)g.idx[c] : (int64_t)c) * ldx;
         }
+    };
+    auto prefetch = [&](int c0) {
+        const T *xc[KC];
+        xcols(c0, xc);
         if (tid < nown) {
-            SV w;
 #pragma unroll
-            for (int cc = 0; cc < KC; cc++) w.v[cc] = xc[cc][r0 + tid];
-            sv[tid] = w;
+            for (int cc = 0; cc < KC; cc++) pown.v[cc] = xc[cc][r0 + tid];
         }
-        for (int j = tid; j < nh; j += kSpmmRows) {
+        if (tid < nh) {
+#pragma unroll
+            for (int cc = 0; cc < KC; cc++) pext.v[cc] = xc[cc][row1];
+        }
+    };
+    prefetch(0);
+    for (int c0 = 0; c0 < k; c0 += KC) {
+        const T *xc[KC];
+        xcols(c0, xc);
+        if (tid < nown) sv[tid] = pown;
+        if (tid < nh) sv[kSpmmRows + tid] = pext;
+        // extras beyond one per thread (blocks with more than 256) load here
+        for (int j = tid + kSpmmRows; j < nh; j += kSpmmRows) {
             const int row = s_h[j];
             SV w;
 #pragma unroll
@@ -452,6 +470,7 @@ __global__ __launch_bounds__(kSpmmRows) void k_spmm_halo(int n, const int *__res
             sv[kSpmmRows + j] = w;
         }
         __syncthreads();
+        if (c0 + KC < k) prefetch(c0 + KC);
         T acc[KC];
 #pragma unroll
         for (int cc = 0; cc < KC; cc++) acc[cc] = T(0);
