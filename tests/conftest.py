@@ -16,5 +16,5 @@ for p in (os.path.join(ROOT, "oracle"), os.path.join(ROOT, "gcge-1.0_amd"), ROOT
 
 
 def pytest_configure(config):
-    config.addinivalue_line("markers", "gpu: needs an MI355X (runs on the GPU box)")
+    config.addinivalue_line("markers", "gpu: needs an MI355X GPU")
     config.addinivalue_line("markers", "slow: long CPU test")
