@@ -982,10 +982,7 @@ void bcg(void *A, void **RHS, void **V, int x_start, int x_length, GCGE_OPS *ops
                 std::fill(b.begin(), b.end(), 0.0);
                 ops->BlockPCGSearchUpdate(A, V, x_start, CG_P, RHS, 0, na, alpha.data(), b.data(), ops);
             } else {
-                // p = r: the copy branch of the search step (beta 0), the same
-                // values as MultiVecAxpby(1, r, 0, p), for any layout of p
-                std::fill(b.begin(), b.end(), 0.0);
-                ops->BlockCGSearchUpdate(V, x_start, CG_P, RHS, 0, na, alpha.data(), b.data(), ops);
+                ops->MultiVecAxpby(1.0, RHS, 0.0, CG_P, ms, me, ops);
             }
         } else {
             for (int id = 0; id < na; id++) b[id] = rho2[id] / rho1[id];
@@ -1051,12 +1048,7 @@ bool bcg32_ready(void *A, int cols, GCGE_OPS *ops, GCGE_WORKSPACE *ws)
     for (int i = 0; i < 4; i++)
         if (ws->bcg32[i]) ops->MultiVecDestroy(&ws->bcg32[i], ws->bcg32_cols, ops);
     const int n = ws->CG_p_size > cols ? ws->CG_p_size : cols;
-    // column 2 holds the search directions: in the SpMM's preferred layout
-    // where the backend offers one
-    for (int i = 0; i < 4; i++) {
-        if (i == 2 && ops->MultiVecCreateCGDirection) ops->MultiVecCreateCGDirection(&ws->bcg32[i], n, A, 1, ops);
-        else ops->MultiVecCreateByMatPrecision(&ws->bcg32[i], n, A, 1, ops);
-    }
+    for (int i = 0; i < 4; i++) ops->MultiVecCreateByMatPrecision(&ws->bcg32[i], n, A, 1, ops);
     ws->bcg32_cols = n;
     return ws->bcg32[0] && ws->bcg32[1] && ws->bcg32[2] && ws->bcg32[3];
 }
@@ -1138,14 +1130,7 @@ void compute_w(void *A, void *B, void **V, double *eval, GCGE_OPS *ops, GCGE_PAR
         ops->MultiVecCopy(ws->bcg32[1], V, ms, me, ops);
     } else if (para->if_use_bcg == 1) {
         if (!pcg_ready(ops, para)) return;
-        // search directions in the SpMM's preferred layout where the backend
-        // offers one (created on first use, kept with the workspace)
-        if (!ws->CG_dir && ops->MultiVecCreateCGDirection) {
-            ops->MultiVecCreateCGDirection(&ws->CG_dir, ws->CG_p_size, A, -1, ops);
-            ws->CG_dir_size = ws->CG_p_size;
-        }
-        bcg(A, ws->V_tmp, V, w_start, w_length, ops, para, ws->CG_dir ? ws->CG_dir : ws->CG_p, ws->evec,
-            ws->subspace_dtmp, ws->subspace_itmp);
+        bcg(A, ws->V_tmp, V, w_start, w_length, ops, para, ws->CG_p, ws->evec, ws->subspace_dtmp, ws->subspace_itmp);
     } else {
         for (int idx = 0; idx < w_length; idx++) {
             void *rhs, *wc;
@@ -1692,7 +1677,6 @@ extern "C" void GCGE_WORKSPACE_Free(GCGE_WORKSPACE **workspace, GCGE_PARA *, GCG
     }
     for (int i = 0; i < 4; i++)
         if (ws->bcg32[i]) ops->MultiVecDestroy(&ws->bcg32[i], ws->bcg32_cols, ops);
-    if (ws->CG_dir) ops->MultiVecDestroy(&ws->CG_dir, ws->CG_dir_size, ops);
     if (ws->mixed32) {
         GCGE_SOLVER *s32 = (GCGE_SOLVER *)ws->mixed32;
         ws->mixed32 = nullptr;

@@ -302,15 +302,7 @@ const int *mat_rowidx(HIP_MAT *A)
 // n = 4,096,000, a multiple of 2^18 bytes per FP64 column).
 static int64_t pad_ld(int64_t n) { return (((n + 255) / 256) | 1) * 256; }
 
-// columns allocated for a multivector: ncols (at least 1), rounded up to a
-// whole group of 4 in the interleaved layout
-static size_t mv_alloc_cols(int ncols, int grp)
-{
-    const size_t c = (size_t)std::max(ncols, 1);
-    return grp == 4 ? (c + 3) / 4 * 4 : c;
-}
-
-static MV *mv_new(int64_t n, int ncols, int prec, HIP_MAT *mat, int grp = 1)
+static MV *mv_new(int64_t n, int ncols, int prec, HIP_MAT *mat)
 {
     if (!ensure_ctx()) return nullptr;
     if (n <= 0 || ncols < 0) {
@@ -323,10 +315,9 @@ static MV *mv_new(int64_t n, int ncols, int prec, HIP_MAT *mat, int grp = 1)
     m->n = n;
     m->ld = pad_ld(n);
     m->ncols = ncols;
-    m->grp = grp;
     m->mat = mat;
     m->data = nullptr;
-    size_t bytes = (size_t)m->ld * mv_alloc_cols(ncols, grp) * elem_size(prec);
+    size_t bytes = (size_t)m->ld * (size_t)std::max(ncols, 1) * elem_size(prec);
     if (!check(hipMalloc(&m->data, bytes), "hipMalloc multivector")) {
         delete m;
         return nullptr;
@@ -350,7 +341,7 @@ static void mv_free(MV *m)
     sync();
     if (m->data) {
         (void)hipFree(m->data);
-        g_ctx.dev_bytes -= (double)((size_t)m->ld * mv_alloc_cols(m->ncols, m->grp) * elem_size(m->prec));
+        g_ctx.dev_bytes -= (double)((size_t)m->ld * (size_t)std::max(m->ncols, 1) * elem_size(m->prec));
     }
     for (int j = 0; j < std::max(m->ncols, 1); j++) m->cols[j].magic = 0;
     delete[] m->cols;
@@ -358,10 +349,7 @@ static void mv_free(MV *m)
     delete m;
 }
 
-// a column range of a multivector; grouped: an interleaved multivector is
-// accepted (by the operations that read its layout) when the range starts on
-// a group
-static bool range_ok(MV *m, int s, int e, const char *what, bool grouped = false)
+static bool range_ok(MV *m, int s, int e, const char *what)
 {
     if (!m) {
         latch(102, "%s: not a device multivector", what);
@@ -369,11 +357,6 @@ static bool range_ok(MV *m, int s, int e, const char *what, bool grouped = false
     }
     if (s < 0 || e < s || e > m->ncols) {
         latch(103, "%s: column range [%d,%d) outside [0,%d)", what, s, e, m->ncols);
-        return false;
-    }
-    if (m->grp != 1 && (!grouped || s % m->grp)) {
-        latch(104, "%s: a multivector of interleaved column groups (the block CG's search directions) %s", what,
-              grouped ? "needs a range that starts on a group" : "is not accepted here");
         return false;
     }
     return true;
@@ -997,12 +980,6 @@ extern "C" GCGE_INT HIP_MultiVecPrecision(void **mv)
     return m ? m->prec : -1;
 }
 
-extern "C" GCGE_INT HIP_MultiVecGroup(void **mv)
-{
-    MV *m = as_mv(mv);
-    return m ? m->grp : -1;
-}
-
 // ====================================================================== GCGE_OPS slots
 namespace {
 
@@ -1351,14 +1328,12 @@ void s_MatDotMultiVecInnerProdDiag(void *mat, void **x, void **y, GCGE_INT *star
 {
     MV *mx = as_mv(x), *my = as_mv(y);
     const int k = end[0] - start[0];
-    if (!range_ok(mx, start[0], end[0], "MatDotMultiVecInnerProdDiag", true) ||
+    if (!range_ok(mx, start[0], end[0], "MatDotMultiVecInnerProdDiag") ||
         !range_ok(my, start[1], start[1] + k, "MatDotMultiVecInnerProdDiag") ||
         !same_rows(mx, my, "MatDotMultiVecInnerProdDiag") || !same_prec(mx, my, "MatDotMultiVecInnerProdDiag") ||
         !mat_ok((HIP_MAT *)mat, mx, "MatDotMultiVecInnerProdDiag"))
         return;
-    // (col_ptr of a group's first column is where the group starts)
-    if (!spmm_dot(mx->prec, (HIP_MAT *)mat, col_ptr(mx, start[0]), mx->ld, col_ptr(my, start[1]), my->ld, k, out,
-                  mx->grp == 4))
+    if (!spmm_dot(mx->prec, (HIP_MAT *)mat, col_ptr(mx, start[0]), mx->ld, col_ptr(my, start[1]), my->ld, k, out))
         for (int j = 0; j < k; j++) out[j] = NAN;
 }
 
@@ -1412,18 +1387,14 @@ void s_BlockPCGSearchUpdate(void *Mat, void **x, GCGE_INT sx, void **p, void **r
         return;
     }
     const int k = std::max(kx, kp);
-    if (!range_ok(mx, sx, sx + kx, who) || !range_ok(mp, 0, k, who, true) || !range_ok(mr, 0, kp, who) ||
+    if (!range_ok(mx, sx, sx + kx, who) || !range_ok(mp, 0, k, who) || !range_ok(mr, 0, kp, who) ||
         !same_rows(mx, mp, who) || !same_rows(mx, mr, who) || !same_prec(mx, mp, who) || !same_prec(mx, mr, who) ||
         !mat_ok(A, mr, who))
         return;
     const void *dinv = mat_dinv(A, mr->prec);
     if (!dinv) return;
-    if (mp->grp == 4)
-        launch_cg_search_il(mx->prec, mx->n, col_ptr(mx, sx), mx->ld, mp->data, mp->ld, mr->data, mr->ld, kx, kp,
-                            alpha, beta, dinv);
-    else
-        launch_cg_search(mx->prec, mx->n, col_ptr(mx, sx), mx->ld, mp->data, mp->ld, mr->data, mr->ld, kx, kp,
-                         alpha, beta, dinv);
+    launch_cg_search(mx->prec, mx->n, col_ptr(mx, sx), mx->ld, mp->data, mp->ld, mr->data, mr->ld, kx, kp, alpha,
+                     beta, dinv);
 }
 
 void s_BlockCGSearchUpdate(void **x, GCGE_INT sx, void **p, void **r, GCGE_INT kx, GCGE_INT kp,
@@ -1435,17 +1406,13 @@ void s_BlockCGSearchUpdate(void **x, GCGE_INT sx, void **p, void **r, GCGE_INT k
         return;
     }
     const int k = std::max(kx, kp);
-    if (!range_ok(mx, sx, sx + kx, "BlockCGSearchUpdate") || !range_ok(mp, 0, k, "BlockCGSearchUpdate", true) ||
+    if (!range_ok(mx, sx, sx + kx, "BlockCGSearchUpdate") || !range_ok(mp, 0, k, "BlockCGSearchUpdate") ||
         !range_ok(mr, 0, kp, "BlockCGSearchUpdate") || !same_rows(mx, mp, "BlockCGSearchUpdate") ||
         !same_rows(mx, mr, "BlockCGSearchUpdate") || !same_prec(mx, mp, "BlockCGSearchUpdate") ||
         !same_prec(mx, mr, "BlockCGSearchUpdate"))
         return;
-    if (mp->grp == 4)
-        launch_cg_search_il(mx->prec, mx->n, col_ptr(mx, sx), mx->ld, mp->data, mp->ld, mr->data, mr->ld, kx, kp,
-                            alpha, beta, nullptr);
-    else
-        launch_cg_search(mx->prec, mx->n, col_ptr(mx, sx), mx->ld, mp->data, mp->ld, mr->data, mr->ld, kx, kp,
-                         alpha, beta);
+    launch_cg_search(mx->prec, mx->n, col_ptr(mx, sx), mx->ld, mp->data, mp->ld, mr->data, mr->ld, kx, kp, alpha,
+                     beta);
 }
 
 void s_MultiVecResidualNorms(void *A, void *B, void **X, const GCGE_INT *idx, GCGE_INT k, const GCGE_DOUBLE *lambda,
@@ -1456,7 +1423,6 @@ void s_MultiVecResidualNorms(void *A, void *B, void **X, const GCGE_INT *idx, GC
         latch(160, "MultiVecResidualNorms: not a device multivector");
         return;
     }
-    if (!range_ok(mx, 0, 0, "MultiVecResidualNorms")) return;
     for (int j = 0; j < k; j++)
         if (idx[j] < 0 || idx[j] >= mx->ncols) {
             latch(161, "MultiVecResidualNorms: column %d outside [0,%d)", idx[j], mx->ncols);
@@ -1518,29 +1484,6 @@ void s_MultiVecCreateByMatPrecision(void ***mv, GCGE_INT n_vec, void *mat, GCGE_
     *mv = (void **)mv_new(A->n, n_vec, precision, A);
 }
 
-// The block CG's search directions p: in groups of 4 interleaved columns
-// when the matrix's SpMM reads that layout (the staged row blocks with 4
-// columns per pass), column-major otherwise or with GCGE_HIP_CG_INTERLEAVE=0.
-// precision < 0: the matrix's current precision.
-void s_MultiVecCreateCGDirection(void ***mv, GCGE_INT n_vec, void *mat, GCGE_INT precision, GCGE_OPS *)
-{
-    HIP_MAT *A = (HIP_MAT *)mat;
-    *mv = nullptr;
-    if (!A || A->magic != kMagicMat) {
-        latch(143, "MultiVecCreateCGDirection: bad arguments");
-        return;
-    }
-    if (precision < 0) precision = A->prec;
-    if (precision != GCGE_HIP_FP64 && precision != GCGE_HIP_FP32) {
-        latch(143, "MultiVecCreateCGDirection: bad precision %d", precision);
-        return;
-    }
-    if (precision == GCGE_HIP_FP32 && !mat_val32(A)) return;
-    const char *e = getenv("GCGE_HIP_CG_INTERLEAVE");
-    const bool il = !(e && std::strcmp(e, "0") == 0) && halo_il_ok(A, precision);
-    *mv = (void **)mv_new(A->n, n_vec, precision, A, il ? 4 : 1);
-}
-
 GCGE_INT s_MultiVecNumRows(void **x)
 {
     MV *m = as_mv(x);
@@ -1592,7 +1535,6 @@ extern "C" void GCGE_HIP_SetOps(GCGE_OPS *ops)
     ops->MultiVecCopy = s_MultiVecCopy;
     ops->MultiVecNumRows = s_MultiVecNumRows;
     ops->MultiVecCreateByMatPrecision = s_MultiVecCreateByMatPrecision;
-    ops->MultiVecCreateCGDirection = s_MultiVecCreateCGDirection;
     GCGE_SetDenseOps(ops);
 }
 

@@ -39,12 +39,6 @@ struct MV {
     int64_t n;        // rows
     int64_t ld;       // leading dimension in elements
     int ncols;
-    // 1: column-major; 4: groups of 4 interleaved columns (row r of group g
-    // holds columns 4g..4g+3 at data + (4g * ld + 4r) elements; ncols rounded
-    // up to a multiple of 4 are allocated).  Only the block CG's search
-    // directions use 4 (MultiVecCreateCGDirection); every other operation
-    // refuses such a multivector.
-    int grp;
     void *data;       // device memory, ld * ncols elements
     Col *cols;        // ncols column views
     HIP_MAT *mat;     // matrix this multivector was created from (may be NULL)
@@ -190,13 +184,9 @@ void launch_zero_rows(int prec, void *x, int64_t ldx, int64_t n, int k, const un
 // Y_j = A X_j (xidx: optional gather of X columns, k <= kMaxCols when given)
 void launch_spmm(int prec, HIP_MAT *A, const void *X, int64_t ldx, void *Y, int64_t ldy, int k,
                  const int *xidx);
-// Y_j = A X_j and out_j = <X_j, Y_j>; with il, X is in groups of 4
-// interleaved columns (MV::grp == 4) and starts on a group
+// Y_j = A X_j and out_j = <X_j, Y_j>
 bool spmm_dot(int prec, HIP_MAT *A, const void *X, int64_t ldx, void *Y, int64_t ldy, int k,
-              double *out, bool il = false);
-// whether A's SpMM can read X in groups of 4 interleaved columns (the staged
-// row blocks with 4 columns per pass, no long rows)
-bool halo_il_ok(const HIP_MAT *A, int prec);
+              double *out);
 // rnorm/xnorm/bxnorm of A x_j - lambda_j B x_j, x_j = X(:, idx[j])
 bool residual_norms(int prec, HIP_MAT *A, HIP_MAT *B, const void *X, int64_t ldx, const int *idx,
                     int k, const double *lambda, double *rnorm, double *xnorm, double *bxnorm);
@@ -215,9 +205,6 @@ void launch_diag_inv(HIP_MAT *A);
 // dinv (not null), p_j = dinv .* r_j + beta_j p_j
 void launch_cg_search(int prec, int64_t n, void *x, int64_t ldx, void *p, int64_t ldp, const void *r, int64_t ldr,
                       int kx, int kp, const double *alpha, const double *beta, const void *dinv = nullptr);
-// launch_cg_search with p in groups of 4 interleaved columns (p starts on a group)
-void launch_cg_search_il(int prec, int64_t n, void *x, int64_t ldx, void *p, int64_t ldp, const void *r,
-                         int64_t ldr, int kx, int kp, const double *alpha, const double *beta, const void *dinv);
 // G (p x q, host, column-major ldg) = V^T W
 bool gram(int prec, int64_t n, const void *V, int64_t ldv, int p, const void *W, int64_t ldw, int q,
           double *G, int ldg, bool upper_only);

@@ -384,11 +384,6 @@ __global__ __launch_bounds__(kSpmmRows) void k_spmm_long(const int *__restrict__
 // bit-identical to k_spmm's.  Rows longer than long_thr are left to
 // k_spmm_long.  Row blocks are numbered so that each XCD walks a contiguous
 // range of rows (a stencil's plane neighbours stay in its L2).
-// With IL, X is stored in groups of 4 interleaved columns (row r of group g
-// holds columns 4g..4g+3 side by side, the group starting where column 4g
-// would in column-major order), so every staged row, own or other, is one
-// 16- or 32-byte load: the isolated rows of a block's faces then use whole
-// fetches instead of one value of each.
 constexpr int kHaloMaxE = 8;
 
 // KC consecutive values of one LDS slot, moved as one vector
@@ -397,7 +392,7 @@ struct alignas(KC * sizeof(T) >= 16 ? 16 : KC * sizeof(T)) SlotVec {
     T v[KC];
 };
 
-template <typename T, bool DOT, int KC, bool IL>
+template <typename T, bool DOT, int KC>
 __global__ __launch_bounds__(kSpmmRows) void k_spmm_halo(int n, const int *__restrict__ rowptr,
                                                          const unsigned short *__restrict__ lcol,
                                                          const T *__restrict__ val, const int *__restrict__ hptr,
@@ -449,12 +444,6 @@ __global__ __launch_bounds__(kSpmmRows) void k_spmm_halo(int n, const int *__res
         }
     };
     auto prefetch = [&](int c0) {
-        if (IL) {  // X in groups of KC = 4 interleaved columns: one vector per row
-            const SV *xg = reinterpret_cast<const SV *>(X + (int64_t)c0 * ldx);
-            if (tid < nown) pown = xg[r0 + tid];
-            if (tid < nh) pext = xg[row1];
-            return;
-        }
         const T *xc[KC];
         xcols(c0, xc);
         if (tid < nown) {
@@ -475,10 +464,6 @@ __global__ __launch_bounds__(kSpmmRows) void k_spmm_halo(int n, const int *__res
         // extras beyond one per thread (blocks with more than 256) load here
         for (int j = tid + kSpmmRows; j < nh; j += kSpmmRows) {
             const int row = s_h[j];
-            if (IL) {
-                sv[kSpmmRows + j] = reinterpret_cast<const SV *>(X + (int64_t)c0 * ldx)[row];
-                continue;
-            }
             SV w;
 #pragma unroll
             for (int cc = 0; cc < KC; cc++) w.v[cc] = xc[cc][row];
@@ -1044,51 +1029,6 @@ __global__ __launch_bounds__(kBlock) void k_cg_search(int64_t n, T *__restrict__
     }
 }
 
-// k_cg_search's step with p in groups of 4 interleaved columns (the layout
-// the staged SpMM reads with IL): a thread takes one row of the 4 columns of
-// its group, p as one vector, x and r column by column; every element sees
-// the operations of k_cg_search in the same order (x from the old p first).
-template <typename T, bool PRE>
-__global__ __launch_bounds__(kBlock) void k_cg_search_il(int64_t n, T *__restrict__ x, int64_t ldx,
-                                                         T *__restrict__ p, const T *__restrict__ r, int64_t ldr,
-                                                         int64_t ldp, int kx, int kp, ColScalars s,
-                                                         const T *__restrict__ dinv)
-{
-    using SV = SlotVec<T, 4>;
-    const int j0 = 4 * (int)blockIdx.y;
-    SV *pg = reinterpret_cast<SV *>(p + (int64_t)j0 * ldp);
-    T al[4], be[4];
-    bool dx[4], dp[4], anyp = false;
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-        dx[c] = j0 + c < kx;
-        dp[c] = j0 + c < kp;
-        anyp = anyp || dp[c];
-        al[c] = dx[c] ? (T)s.a[j0 + c] : T(0);
-        be[c] = dp[c] ? (T)s.b[j0 + c] : T(0);
-    }
-    const int64_t stride = (int64_t)gridDim.x * kBlock;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
-        SV pv = pg[i];
-#pragma unroll
-        for (int c = 0; c < 4; c++)
-            if (dx[c]) {
-                T *xj = x + (int64_t)(j0 + c) * ldx;
-                xj[i] = axpby_el<T>(al[c], pv.v[c], T(1), xj[i]);
-            }
-        if (anyp) {
-            const T d = PRE ? dinv[i] : T(1);
-#pragma unroll
-            for (int c = 0; c < 4; c++)
-                if (dp[c]) {
-                    const T rv = r[(int64_t)(j0 + c) * ldr + i];
-                    pv.v[c] = axpby_el<T>(T(1), PRE ? rv * d : rv, be[c], pv.v[c]);
-                }
-            pg[i] = pv;
-        }
-    }
-}
-
 // ---------------------------------------------------------------- gradient
 // g_e = sum_i w_i x_i[row_e] x_i[col_e], one thread per stored entry
 template <typename T>
@@ -1343,19 +1283,19 @@ static int halo_kc(const HIP_MAT *A, size_t elem)
     return kc;
 }
 
-template <typename T, bool DOT, int KC, bool IL = false>
+template <typename T, bool DOT, int KC>
 static void spmm_halo_kc(HIP_MAT *A, const T *val, const T *Xc, int64_t ldx, ColScalars g, T *Yc, int64_t ldy,
                          int kc, double *partial, int nblk, int long_thr, int pstride)
 {
     const size_t lds = (size_t)KC * A->halo_cap * sizeof(T) + (size_t)(A->halo_cap - kSpmmRows) * sizeof(int);
-    k_spmm_halo<T, DOT, KC, IL><<<((nblk + 7) / 8) * 8, kSpmmRows, lds, ctx().stream>>>(
+    k_spmm_halo<T, DOT, KC><<<((nblk + 7) / 8) * 8, kSpmmRows, lds, ctx().stream>>>(
         A->n, A->rowptr, A->lcol, val, A->halo_ptr, A->halo_col, Xc, ldx, Yc, ldy, kc, g, partial, nblk, long_thr,
         pstride, A->halo_cap);
 }
 
 template <typename T, bool DOT>
 static bool spmm_halo(HIP_MAT *A, const T *val, const T *Xc, int64_t ldx, ColScalars g, T *Yc, int64_t ldy,
-                      int kc, double *out, bool il = false)
+                      int kc, double *out)
 {
     const int nblk = (A->n + kSpmmRows - 1) / kSpmmRows;
     const int nlong = A->nlong, long_thr = nlong > 0 ? kLongRow : INT_MAX;
@@ -1370,9 +1310,7 @@ static bool spmm_halo(HIP_MAT *A, const T *val, const T *Xc, int64_t ldx, ColSca
         const bool f64 = sizeof(T) == 8;
         ProfScope ps(DOT ? (f64 ? K_SPMM_DOT_F64 : K_SPMM_DOT_F32) : (f64 ? K_SPMM_F64 : K_SPMM_F32), bytes,
                      2.0 * kc * (double)A->nnz);
-        if (il) {
-            spmm_halo_kc<T, DOT, 4, true>(A, val, Xc, ldx, g, Yc, ldy, kc, partial, nblk, long_thr, pstride);
-        } else switch (halo_kc(A, sizeof(T))) {
+        switch (halo_kc(A, sizeof(T))) {
         case 8: spmm_halo_kc<T, DOT, 8>(A, val, Xc, ldx, g, Yc, ldy, kc, partial, nblk, long_thr, pstride); break;
         case 4: spmm_halo_kc<T, DOT, 4>(A, val, Xc, ldx, g, Yc, ldy, kc, partial, nblk, long_thr, pstride); break;
         case 2: spmm_halo_kc<T, DOT, 2>(A, val, Xc, ldx, g, Yc, ldy, kc, partial, nblk, long_thr, pstride); break;
@@ -1388,27 +1326,16 @@ static bool spmm_halo(HIP_MAT *A, const T *val, const T *Xc, int64_t ldx, ColSca
 
 template <typename T, bool DOT>
 static bool spmm_t(HIP_MAT *A, const void *X, int64_t ldx, void *Y, int64_t ldy, int k, const int *xidx,
-                   double *out, bool il = false)
+                   double *out)
 {
     const int nblk = (A->n + kSpmmRows - 1) / kSpmmRows;
     const T *val = mat_vals<T>(A);
     if (!val) return false;
-    if (il && !halo_il_ok(A, sizeof(T) == 8 ? GCGE_HIP_FP64 : GCGE_HIP_FP32)) {
-        latch(176, "SpMM: a multivector of interleaved column groups needs the staged-row-block SpMM "
-                   "with 4 columns per pass and no long rows");
-        return false;
-    }
     for (int c0 = 0; c0 < k; c0 += kMaxCols) {
         const int kc = std::min(kMaxCols, k - c0);
         ColScalars g = make_scalars(kc, nullptr, nullptr, xidx ? xidx + c0 : nullptr);
-        // (c0 is a multiple of 4: in the interleaved layout too, X + c0 * ldx
-        // is where the group of column c0 starts)
         const T *Xc = (const T *)X + (xidx ? 0 : (int64_t)c0 * ldx);
         T *Yc = (T *)Y + (int64_t)c0 * ldy;
-        if (il) {
-            if (!spmm_halo<T, DOT>(A, val, Xc, ldx, g, Yc, ldy, kc, DOT ? out + c0 : nullptr, true)) return false;
-            continue;
-        }
         if (A->gather_rows && kc >= 2) {  // one column gathers a line per entry either way
             if (!spmm_rows<T, DOT>(A, val, Xc, ldx, g, Yc, ldy, kc, DOT ? out + c0 : nullptr)) return false;
             continue;
@@ -1458,18 +1385,12 @@ void launch_spmm(int prec, HIP_MAT *A, const void *X, int64_t ldx, void *Y, int6
     else spmm_t<float, false>(A, X, ldx, Y, ldy, k, xidx, nullptr);
 }
 
-bool spmm_dot(int prec, HIP_MAT *A, const void *X, int64_t ldx, void *Y, int64_t ldy, int k, double *out, bool il)
+bool spmm_dot(int prec, HIP_MAT *A, const void *X, int64_t ldx, void *Y, int64_t ldy, int k, double *out)
 {
     if (k <= 0) return true;
     if (failed()) return false;
-    if (prec == GCGE_HIP_FP64) return spmm_t<double, true>(A, X, ldx, Y, ldy, k, nullptr, out, il);
-    return spmm_t<float, true>(A, X, ldx, Y, ldy, k, nullptr, out, il);
-}
-
-bool halo_il_ok(const HIP_MAT *A, int prec)
-{
-    return A->halo && !A->gather_rows && A->nlong == 0 &&
-           halo_kc(A, prec == GCGE_HIP_FP32 ? sizeof(float) : sizeof(double)) == 4;
+    if (prec == GCGE_HIP_FP64) return spmm_t<double, true>(A, X, ldx, Y, ldy, k, nullptr, out);
+    return spmm_t<float, true>(A, X, ldx, Y, ldy, k, nullptr, out);
 }
 
 // Residual sums of kc <= kMaxCols columns on the row-major gather path;
@@ -1707,42 +1628,6 @@ void launch_cg_search(int prec, int64_t n, void *x, int64_t ldx, void *p, int64_
 #undef GH_SEARCH
         }
         if (!check(hipGetLastError(), "k_cg_search")) return;
-    }
-}
-
-void launch_cg_search_il(int prec, int64_t n, void *x, int64_t ldx, void *p, int64_t ldp, const void *r,
-                         int64_t ldr, int kx, int kp, const double *alpha, const double *beta, const void *dinv)
-{
-    const int k = std::max(kx, kp);
-    if (k <= 0 || failed()) return;
-    const int nb = grid_1d(n, 1024);
-    for (int c0 = 0; c0 < k; c0 += kMaxCols) {  // kMaxCols is a multiple of 4: chunks start on groups
-        const int kc = std::min(kMaxCols, k - c0);
-        const int kxc = std::max(0, std::min(kc, kx - c0)), kpc = std::max(0, std::min(kc, kp - c0));
-        std::vector<double> aa(kc, 0.0), bb(kc, 0.0);
-        for (int j = 0; j < kxc; j++) aa[j] = alpha[c0 + j];
-        for (int j = 0; j < kpc; j++) bb[j] = beta[c0 + j];
-        ColScalars s = make_scalars(kc, aa.data(), bb.data(), nullptr);
-        dim3 grid(nb, (kc + 3) / 4);
-        {
-            ProfScope ps(prec == GCGE_HIP_FP64 ? K_CG_SEARCH_F64 : K_CG_SEARCH_F32,
-                         (3.0 * kxc + 3.0 * kpc - 1.0 * std::min(kxc, kpc)) * (double)n * elem_size(prec) +
-                             (dinv && kpc > 0 ? (double)n * elem_size(prec) : 0.0),
-                         0.0);
-#define GH_SEARCH_IL(T_, PRE_)                                                                                    \
-    k_cg_search_il<T_, PRE_><<<grid, kBlock, 0, ctx().stream>>>(                                                 \
-        n, (T_ *)x + (int64_t)c0 * ldx, ldx, (T_ *)p + (int64_t)c0 * ldp, (const T_ *)r + (int64_t)c0 * ldr, ldr, \
-        ldp, kxc, kpc, s, (const T_ *)dinv)
-            if (prec == GCGE_HIP_FP64) {
-                if (dinv) GH_SEARCH_IL(double, true);
-                else GH_SEARCH_IL(double, false);
-            } else {
-                if (dinv) GH_SEARCH_IL(float, true);
-                else GH_SEARCH_IL(float, false);
-            }
-#undef GH_SEARCH_IL
-        }
-        if (!check(hipGetLastError(), "k_cg_search_il")) return;
     }
 }
 

@@ -77,10 +77,6 @@ extern "C" GCGE_INT GCGE_HIP_MultiVecInnerProdDiagNoAlloc(void **x, void **y, GC
         latch(202, "MultiVecInnerProdDiagNoAlloc: %d columns exceed the limit of %d per call", k, kNoAllocMaxCols);
         return 202;
     }
-    if (mx->grp != 1 || my->grp != 1) {
-        latch(206, "MultiVecInnerProdDiagNoAlloc: interleaved column groups are not accepted");
-        return 206;
-    }
     if (mx->n != my->n || mx->prec != my->prec) {
         latch(203, "MultiVecInnerProdDiagNoAlloc: multivectors differ in rows or precision");
         return 203;

@@ -148,11 +148,6 @@ class MultiVec:
         _check("HIP_MultiVecGetToHost")
         return X
 
-    @property
-    def group(self):
-        """1: column-major; 4: groups of 4 interleaved columns."""
-        return capi.load().HIP_MultiVecGroup(self.ptr)
-
     def close(self):
         if self.ptr:
             p = C.c_void_p(self.ptr)
@@ -192,19 +187,6 @@ class Ops:
 
     def _f(self, name, ftype):
         return C.cast(getattr(self.o, name), ftype)
-
-    def cg_direction(self, A: Matrix, ncols, prec=capi.GCGE_HIP_FP64):
-        """The MultiVecCreateCGDirection slot: the block CG's search-direction
-        block, in the layout A's SpMM reads fastest (MultiVec.group: 4 for
-        interleaved column groups, which only matdot_dot and the search-step
-        slots accept)."""
-        h = C.c_void_p()
-        self._f("MultiVecCreateCGDirection", capi.F_MultiVecCreateCGDirection)(C.byref(h), ncols, A.ptr, prec,
-                                                                               self._p)
-        _check("MultiVecCreateCGDirection")
-        m = MultiVec.__new__(MultiVec)
-        m.ptr, m.n, m.ncols, m.prec = h.value, A.n, ncols, prec
-        return m
 
     def matdot(self, A: Matrix, X: MultiVec, Y: MultiVec, xs=0, ys=0, k=None):
         k = X.ncols - xs if k is None else k

@@ -80,7 +80,7 @@ class GCGE_WORKSPACE(C.Structure):
         ("num_soft_locked_in_last_iter", c_int), ("dense_matrix", c_void_p),
         ("subspace_matrix", DP), ("subspace_evec", DP), ("subspace_dtmp", DP), ("subspace_itmp", IP),
         ("eval", DP), ("orth_ind", IP), ("bcg32", c_void_p * 4), ("bcg32_cols", c_int),
-        ("mixed32", c_void_p), ("CG_dir", c_void_p), ("CG_dir_size", c_int),
+        ("mixed32", c_void_p),
     ]
 
 
@@ -97,7 +97,7 @@ OPS_SLOTS = [
     "MultiVecAxpbyDiag", "MultiVecInnerProdDiag", "MultiVecLinearCombAxpby", "MatDotMultiVecInnerProdDiag",
     "BlockCGUpdate", "MultiVecResidualNorms", "MultiVecGather", "MultiVecCopy", "MultiVecNumRows",
     "MultiVecCreateByMatPrecision", "BlockCGResidualUpdate", "BlockCGSearchUpdate", "BlockPCGResidualUpdate",
-    "BlockPCGSearchUpdate", "MultiVecCreateCGDirection",
+    "BlockPCGSearchUpdate",
 ]
 
 
@@ -183,7 +183,6 @@ SIGNATURES = {
     "HIP_MultiVecGetToHost": (c_int, [c_void_p, c_int, c_int, DP, c_int]),
     "HIP_MultiVecNumCols": (c_int, [c_void_p]),
     "HIP_MultiVecPrecision": (c_int, [c_void_p]),
-    "HIP_MultiVecGroup": (c_int, [c_void_p]),
     "GCGE_HIP_SetOps": (None, [C.POINTER(GCGE_OPS)]),
     "GCGE_SOLVER_SetHIPOps": (None, [SP]),
     "GCGE_HIP_Solver_Init": (SP, [c_void_p, c_void_p, c_int, c_int, C.POINTER(c_char_p)]),
@@ -219,7 +218,6 @@ F_BlockCGSearchUpdate = C.CFUNCTYPE(None, c_void_p, c_int, c_void_p, c_void_p, c
 F_BlockPCGResidualUpdate = C.CFUNCTYPE(None, c_void_p, c_void_p, c_void_p, c_int, DP, DP, DP, c_void_p)
 F_BlockPCGSearchUpdate = C.CFUNCTYPE(None, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_int, DP, DP,
                                      c_void_p)
-F_MultiVecCreateCGDirection = C.CFUNCTYPE(None, C.POINTER(c_void_p), c_int, c_void_p, c_int, c_void_p)
 F_MultiVecResidualNorms = C.CFUNCTYPE(None, c_void_p, c_void_p, c_void_p, IP, c_int, DP, DP, DP, DP, c_void_p)
 F_MultiVecGather = C.CFUNCTYPE(None, c_void_p, IP, c_int, c_void_p, c_int, c_void_p)
 F_MultiVecCopy = C.CFUNCTYPE(None, c_void_p, c_void_p, IP, IP, c_void_p)

@@ -99,11 +99,6 @@ GCGE_INT HIP_MultiVecGetToHost(void **mv, GCGE_INT start, GCGE_INT ncols, GCGE_D
                                GCGE_INT ldh);
 GCGE_INT HIP_MultiVecNumCols(void **mv);
 GCGE_INT HIP_MultiVecPrecision(void **mv);
-/* the layout: 1 column-major; 4 groups of 4 interleaved columns (the block
- * CG's search directions from the MultiVecCreateCGDirection slot, which only
- * MatDotMultiVecInnerProdDiag and the two search-step slots accept; no
- * counterpart in the reference) */
-GCGE_INT HIP_MultiVecGroup(void **mv);
 
 /* ---- registration (counterparts of gcge_app_csr.h:81-84) ---- */
 

@@ -156,13 +156,6 @@ typedef struct GCGE_OPS_ {
     void (*BlockPCGSearchUpdate)(void *Matrix, void **x, GCGE_INT sx, void **p, void **r, GCGE_INT kx,
                                  GCGE_INT kp, const GCGE_DOUBLE *alpha, const GCGE_DOUBLE *beta,
                                  struct GCGE_OPS_ *ops);
-    /* the block CG's search-direction block p (n_vec columns; precision < 0:
-     * the matrix's), in whatever layout the backend's SpMM reads fastest.
-     * Such a block is passed only to MatDotMultiVecInnerProdDiag (as x, from
-     * column 0) and to the two search steps (as p).  Optional: without it the
-     * block CG uses CG_p from MultiVecCreateByMat. */
-    void (*MultiVecCreateCGDirection)(void ***multi_vec, GCGE_INT n_vec, void *mat, GCGE_INT precision,
-                                      struct GCGE_OPS_ *ops);
 } GCGE_OPS;
 
 void GCGE_OPS_Create(GCGE_OPS **ops);

@@ -65,9 +65,6 @@ typedef struct GCGE_WORKSPACE_ {
     GCGE_INT bcg32_cols; /* columns allocated in each of them                        */
     void *mixed32;       /* solver of the FP32 phase of GCGE_HIP_SOLVER_SolveMixed,
                             kept for later calls (NULL until the first)              */
-    void **CG_dir;       /* search directions of the FP64 block CG from
-                            MultiVecCreateCGDirection (NULL: CG_p is used)           */
-    GCGE_INT CG_dir_size;
 } GCGE_WORKSPACE;
 
 typedef struct GCGE_SOLVER_ {

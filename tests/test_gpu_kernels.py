@@ -529,64 +529,6 @@ def test_pcg_steps(ops, prec):
     np.testing.assert_array_equal(p.get(), p_ref.astype(np.float64))
 
 
-@pytest.mark.parametrize("prec", ["f64", "f32"])
-@pytest.mark.parametrize("k", [4, 7, 13])
-def test_cg_directions_interleaved(ops, prec, k):
-    """The block CG's search directions from MultiVecCreateCGDirection on a
-    matrix with the staged-row-block SpMM come in groups of 4 interleaved
-    columns.  The search steps (Jacobi and plain, kx and kp in both
-    relations, partial groups) and the SpMM with fused dots give the same
-    bits as with column-major directions; in FP64 the SpMM of p = D^-1 r is
-    the oracle's CSR product bit for bit (caller's order, locality order on
-    the device); every other slot refuses the layout; with
-    GCGE_HIP_CG_INTERLEAVE=0 the slot returns a column-major block."""
-    A, _ = P.vardiff3d(24, seed=3)
-    mA = Matrix(A, order="locality")
-    assert capi.load().HIP_MatGatherKind(mA.ptr) == 2
-    pr = F32 if prec == "f32" else F64
-    n = A.shape[0]
-    g = rng(61 + k)
-    R, X = g.standard_normal((n, k)), g.standard_normal((n, k))
-    al, be = g.standard_normal(k), g.standard_normal(k)
-    pi = ops.cg_direction(mA, k, pr)
-    pc = MultiVec(mA, k, pr)
-    assert (pi.group, pc.group) == (4, 1)
-    res = []
-    for p in (pi, pc):
-        r, x, w = MultiVec(mA, k, pr, data=R), MultiVec(mA, k, pr, data=X), MultiVec(mA, k, pr)
-        ops.pcg_search_update(mA, x, 0, p, r, 0, k, al, np.zeros(k))  # p = D^-1 r
-        d1 = ops.matdot_dot(mA, p, w)
-        w1 = w.get()
-        ops.pcg_search_update(mA, x, 0, p, r, k - 2, k, al, be)  # x += al p on k-2 columns, p = z + be p
-        d2 = ops.matdot_dot(mA, p, w)
-        w2 = w.get()
-        ops.cg_search_update(x, 0, p, r, k, k - 1, al, be)  # plain CG step, kx > kp
-        d3 = ops.matdot_dot(mA, p, w)
-        res.append((x.get(), w1, w2, w.get(), d1, d2, d3))
-        for m in (r, x, w):
-            m.close()
-    for a, b in zip(*res):
-        np.testing.assert_array_equal(a, b)
-    if pr == F64:
-        d = A.diagonal()
-        np.testing.assert_array_equal(res[0][1], O.csr_matmultivec(A, R * (1.0 / d)[:, None]))
-    with pytest.raises(RuntimeError, match="interleaved"):
-        pi.get()
-    x = MultiVec(mA, k, pr)
-    with pytest.raises(RuntimeError, match="interleaved"):
-        ops.axpby(1.0, pi, 0.0, x)
-    for m in (pi, pc, x):
-        m.close()
-    os.environ["GCGE_HIP_CG_INTERLEAVE"] = "0"
-    try:
-        pz = ops.cg_direction(mA, k, pr)
-        assert pz.group == 1
-        pz.close()
-    finally:
-        del os.environ["GCGE_HIP_CG_INTERLEAVE"]
-    mA.close()
-
-
 @pytest.mark.parametrize("withB", [False, True])
 def test_residual_norms(ops, withB):
     A, B = P.lap3d_mass(9)

@@ -20,7 +20,7 @@ import scipy.sparse as sp
 
 import oracle_api as O
 from gcge_hip import problems as P
-from gcge_hip import Matrix, MultiVec, Ops, Solver, capi, solve
+from gcge_hip import Matrix, MultiVec, Ops, Solver, solve
 
 pytestmark = pytest.mark.gpu
 
@@ -632,38 +632,6 @@ def test_generalized_full_size_workload_matches_reference(order):
     assert ns_residual(A, B, rm["evec"], rm["eval"]).max() <= 1e-8 * (1 + 1e-6)
     mA.close()
     mB.close()
-
-
-@pytest.mark.parametrize("mode", ["fp64", "fp64-jacobi", "mixed"])
-def test_interleaved_cg_directions_give_identical_solves(monkeypatch, mode):
-    """Where the SpMM reads it (staged row blocks, 4 columns per pass), the
-    block CG keeps its search directions in groups of 4 interleaved columns.
-    That changes only where p is stored: the solve is bit for bit the one
-    with column-major directions (GCGE_HIP_CG_INTERLEAVE=0), in the FP64
-    drop-in, with Jacobi-preconditioned inner solves and in the mixed path
-    (same eigenvalues, eigenvectors and iteration count)."""
-    import ctypes as C
-    A, _ = P.vardiff3d(24, seed=5)
-    n, nev = A.shape[0], 12
-    X0 = P.random_x0(n, nev, 3)
-    kw = dict(conv_type="NS", ev_tol=1e-8, ev_max_it=500)
-    if mode == "fp64-jacobi":
-        kw.update(w_orth_type="cholqr", cg_max_it=30, bcg_precond=1)
-    out = []
-    for il in ("1", "0"):
-        monkeypatch.setenv("GCGE_HIP_CG_INTERLEAVE", il)
-        mA = Matrix(A, order="locality")
-        S = Solver(mA, None, nev, precision="mixed" if mode == "mixed" else "fp64", **kw)
-        r = S.solve(X0)
-        ws = S.solver.contents.workspace.contents
-        dirs = ws.bcg32[2] if mode == "mixed" else ws.CG_dir
-        assert dirs and capi.load().HIP_MultiVecGroup(C.c_void_p(dirs)) == (4 if il == "1" else 1)
-        out.append((np.asarray(r["eval"]), r["num_iter"], np.asarray(r["evec"])))
-        S.close()
-        mA.close()
-    np.testing.assert_array_equal(out[0][0], out[1][0])
-    assert out[0][1] == out[1][1]
-    np.testing.assert_array_equal(out[0][2], out[1][2])
 
 
 def test_mixed_full_size_3d_diffusion_matches_fp64():
